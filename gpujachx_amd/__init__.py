@@ -1,0 +1,196 @@
+"""gpujachx_amd -- blocked one-sided (hyperbolic) Jacobi SVD on MI355X (gfx950).
+
+The product is the native library ``lib/libhypjac.so`` (HIP kernels + C ABI,
+see ``include/hypjac.h``).  This module is a thin ctypes mirror of the
+reference's interface (``hypJacL2``, ``init_strats``, ``configureGPU``) with
+numpy buffers, plus access to the library's additions.  It never falls back to
+a CPU implementation: if the library is missing, importing raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libhypjac.so")
+STRAT_SO = os.path.join(LIB_DIR, "strat.so")
+
+# ALG bits (share/HypJacL.hpp:88-110)
+BLK_ORI = 8
+ACCUMV_2 = 4
+FULL_SVD = 2
+CDSORT_1 = 1
+NSWEEP = 30
+
+FAMILIES = ("mmstep", "BrentL", "colcyc", "cycloc", "rowcyc", "cycwor")
+
+_lib = None
+
+
+def lib():
+    """Load libhypjac.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libhypjac.so is missing (%s): build it with `python -m gpujachx_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    u, i, d = ctypes.c_uint, ctypes.c_int, ctypes.c_double
+    pd, pu = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint)
+    pull = ctypes.POINTER(ctypes.c_ulonglong)
+    vp, cp = ctypes.c_void_p, ctypes.c_char_p
+    L.hypJacL2.argtypes = [u, u, u, u, pd, u, pd, u, pd, pu, pull, pull, pd]
+    L.hypJacL2.restype = i
+    L.hypJacL2_dev.argtypes = [u, u, u, u, vp, u, vp, u, vp, pu, pull, pull, pd, vp]
+    L.hypJacL2_dev.restype = i
+    L.init_strats.argtypes = [cp, cp, u, cp, u]
+    L.init_strats.restype = None
+    L.hj_init_strats_builtin.argtypes = [cp, u, cp, u]
+    L.hj_init_strats_builtin.restype = None
+    L.hj_strat_generate.argtypes = [cp, u, ctypes.POINTER(ctypes.c_ushort), pu]
+    L.hj_strat_generate.restype = i
+    L.configureGPU.argtypes = [i]
+    L.configureGPU.restype = i
+    L.hj_last_sweeps.argtypes = [pu, pu, pu, pd]
+    L.hj_last_sweeps.restype = None
+    L.hj_set_profile.argtypes = [i]
+    L.hj_set_profile.restype = None
+    L.hj_kernel_ms.argtypes = [i, pull]
+    L.hj_kernel_ms.restype = d
+    L.hj_gram_nchunk.argtypes = [u, u]
+    L.hj_gram_nchunk.restype = u
+    L.hj_set_step_limit.argtypes = [ctypes.c_long]
+    L.hj_set_step_limit.restype = None
+    L.hj_dbg_scalar.argtypes = [i, i, pd, pd, pd, pd, pd, ctypes.POINTER(ctypes.c_int)]
+    L.hj_dbg_scalar.restype = i
+    L.hj_version.argtypes = []
+    L.hj_version.restype = cp
+    _lib = L
+    return L
+
+
+def _pd(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def strat_generate(fam, n):
+    """Built-in table of family `fam`, order n: uint16 array [steps, n/2, 2]."""
+    L = lib()
+    steps = ctypes.c_uint(0)
+    if L.hj_strat_generate(fam.encode(), n, None, ctypes.byref(steps)) != 0:
+        raise ValueError("no construction for %s %d" % (fam, n))
+    out = np.zeros((steps.value, n // 2, 2), dtype=np.uint16)
+    L.hj_strat_generate(fam.encode(), n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort)),
+                        ctypes.byref(steps))
+    return out
+
+
+def init_strats(sdy, snp0, n0, snp1, n1):
+    """share/HypJacL.hpp:206 -- load tables from a strat.so (fail-stop)."""
+    lib().init_strats(sdy.encode(), snp0.encode(), n0, snp1.encode(), n1)
+
+
+def init_strats_builtin(snp0, n0, snp1, n1):
+    """Same tables from the library's own generator (no strat.so)."""
+    lib().hj_init_strats_builtin(snp0.encode(), n0, snp1.encode(), n1)
+
+
+def current_strats():
+    """(strat0 [S0,16,2] uint8, strat1 [S1,P1,2] uint16) currently loaded."""
+    L = lib()
+    s0 = ctypes.c_uint.in_dll(L, "STRAT0_STEPS").value
+    p0 = ctypes.c_uint.in_dll(L, "STRAT0_PAIRS").value
+    s1 = ctypes.c_uint.in_dll(L, "STRAT1_STEPS").value
+    p1 = ctypes.c_uint.in_dll(L, "STRAT1_PAIRS").value
+    t0 = np.ctypeslib.as_array((ctypes.c_ubyte * (32 * 16 * 2)).in_dll(L, "strat0")).reshape(32, 16, 2)
+    t1 = np.ctypeslib.as_array((ctypes.c_ushort * (1024 * 512 * 2)).in_dll(L, "strat1")).reshape(1024, 512, 2)
+    return t0[:s0, :p0].copy(), t1[:s1, :p1].copy()
+
+
+def strat_ids():
+    L = lib()
+    return ctypes.c_uint.in_dll(L, "STRAT0").value, ctypes.c_uint.in_dll(L, "STRAT1").value
+
+
+def configureGPU(dev=0):
+    return lib().configureGPU(dev)
+
+
+class Result:
+    """Outputs of one hypJacL2 call."""
+
+    def __init__(self, ret, G, V, D, sweeps, rot_s, rot_b, timing):
+        self.ret, self.G, self.V, self.D = ret, G, V, D
+        self.sweeps, self.rot_s, self.rot_b, self.timing = sweeps, rot_s, rot_b, timing
+
+    @property
+    def sigma(self):
+        return np.sqrt(np.abs(self.D))
+
+
+def hypJacL2(routine, G, nplus=None, ldhG=None, want_v=None):
+    """1/HypJacL2.hpp:6-27 on numpy data.  G (nrow x ncol, float64) is copied
+    into a Fortran-ordered buffer with leading dimension ldhG and overwritten
+    by U; returns Result (ret < 0 on an argument error, like the reference)."""
+    L = lib()
+    G = np.asarray(G, dtype=np.float64)
+    nrow, ncol = G.shape
+    nplus = ncol if nplus is None else nplus
+    ldhG = nrow if ldhG is None else ldhG
+    hG = np.zeros((ncol, ldhG), dtype=np.float64)
+    hG[:, :nrow] = G.T
+    full = bool(routine & FULL_SVD) if want_v is None else want_v
+    hV = np.zeros((ncol, ncol), dtype=np.float64) if full else None
+    hD = np.zeros(ncol, dtype=np.float64)
+    sw = ctypes.c_uint(0)
+    gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    tm = np.zeros(4, dtype=np.float64)
+    ret = L.hypJacL2(routine, nrow, ncol, nplus, _pd(hG), ldhG, _pd(hV), ncol if full else 0,
+                     _pd(hD), ctypes.byref(sw), ctypes.byref(gs), ctypes.byref(gb), _pd(tm))
+    return Result(ret, hG[:, :nrow].T.copy(), None if hV is None else hV.T.copy(), hD,
+                  sw.value, gs.value, gb.value, tm)
+
+
+def last_sweeps():
+    L = lib()
+    n = ctypes.c_uint(0)
+    s = (ctypes.c_uint * NSWEEP)()
+    b = (ctypes.c_uint * NSWEEP)()
+    t = (ctypes.c_double * NSWEEP)()
+    L.hj_last_sweeps(ctypes.byref(n), s, b, t)
+    k = n.value
+    return list(s[:k]), list(b[:k]), list(t[:k])
+
+
+def kernel_ms(kind):
+    n = ctypes.c_ulonglong(0)
+    ms = lib().hj_kernel_ms(kind, ctypes.byref(n))
+    return ms, n.value
+
+
+def gram_nchunk(nrow, npairs):
+    return lib().hj_gram_nchunk(nrow, npairs)
+
+
+def set_step_limit(steps):
+    lib().hj_set_step_limit(int(steps))
+
+
+def set_profile(on):
+    lib().hj_set_profile(1 if on else 0)
+
+
+def dbg_scalar(kind, a, b=None, c=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.size
+    b = np.ascontiguousarray(a if b is None else b, dtype=np.float64)
+    c = np.ascontiguousarray(a if c is None else c, dtype=np.float64)
+    oc, ot = np.zeros(n), np.zeros(n)
+    ob = np.zeros(n, dtype=np.int32)
+    rc = lib().hj_dbg_scalar(kind, n, _pd(a), _pd(b), _pd(c), _pd(oc), _pd(ot),
+                             ob.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc:
+        raise RuntimeError("hj_dbg_scalar failed")
+    return oc, ot, ob

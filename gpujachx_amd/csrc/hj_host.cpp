@@ -1,0 +1,524 @@
+// hj_host.cpp -- level-2 driver (hypJacL2), the device-resident variant, the
+// thin level-1 layer and device configuration, on the HIP runtime.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hypjac.h"
+#include "hj_kernels.hpp"
+#include "hj_strat.hpp"
+#include "hj_util.hpp"
+
+namespace hj {
+
+struct Stats {
+  unsigned nswp = 0u;
+  unsigned rot_s[HYPJAC_NSWEEP] = {0u};
+  unsigned rot_b[HYPJAC_NSWEEP] = {0u};
+  double sec[HYPJAC_NSWEEP] = {0.0};
+  double kms[3] = {0.0, 0.0, 0.0};
+  unsigned long long kcnt[3] = {0ull, 0ull, 0ull};
+};
+static Stats g_stats;
+static int g_profile = 0;
+static long g_step_limit = 0;
+
+static bool quiet()
+{
+  const char *q = getenv("HJ_QUIET");
+  return q && *q && *q != '0';
+}
+
+unsigned gram_nchunk(unsigned nrow, unsigned npairs)
+{
+  unsigned target = npairs ? (2048u / npairs) : 1u;
+  if (target > 16u) target = 16u;
+  if (!target) target = 1u;
+  const unsigned g = nrow / 64u;
+  unsigned nc = 1u;
+  while ((nc * 2u) <= target && g && !(g % (nc * 2u)))
+    nc *= 2u;
+  return nc;
+}
+
+// rows per wave of k_update: at least ~8192 waves per step, 64..1024 rows
+static unsigned update_rows(unsigned nrow, unsigned nV, unsigned np)
+{
+  const unsigned long long total = (unsigned long long)np * (nrow + nV);
+  unsigned ch = 64u;
+  while (ch < 1024u && total / ch > 8192ull)
+    ch *= 2u;
+  return ch;
+}
+
+// Device workspace of one level-2 run.
+struct Work {
+  unsigned np = 0u, steps = 0u, s0steps = 0u, nchunk = 0u, CH = 0u, chG = 0u, chV = 0u;
+  uint16_t *d_s1 = nullptr;
+  uint8_t *d_s0 = nullptr;
+  double *d_part = nullptr, *d_W = nullptr;
+  unsigned *d_flags = nullptr, *d_pstat = nullptr;
+  unsigned long long *d_cvg = nullptr, *h_cvg = nullptr;
+
+  void alloc(unsigned nrow, unsigned ncol, bool withV)
+  {
+    np = ncol / 32u;
+    steps = STRAT1_STEPS;
+    s0steps = STRAT0_STEPS;
+    nchunk = gram_nchunk(nrow, np);
+    CH = nrow / nchunk;
+    chG = update_rows(nrow, withV ? ncol : 0u, np);
+    chV = chG;
+    HIP_CALL(hipMalloc(&d_s1, sizeof(uint16_t) * 2u * np * steps));
+    HIP_CALL(hipMalloc(&d_s0, STRAT0_TAB_BYTES));
+    HIP_CALL(hipMalloc(&d_part, sizeof(double) * 768u * nchunk * np));
+    HIP_CALL(hipMalloc(&d_W, sizeof(double) * 1024u * np));
+    HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np));
+    HIP_CALL(hipMalloc(&d_pstat, sizeof(unsigned) * 2u * np));
+    HIP_CALL(hipMalloc(&d_cvg, sizeof(unsigned long long)));
+    HIP_CALL(hipHostMalloc(&h_cvg, sizeof(unsigned long long), hipHostMallocDefault));
+    std::vector<uint16_t> s1((size_t)2u * np * steps);
+    for (unsigned s = 0u; s < steps; ++s)
+      for (unsigned p = 0u; p < np; ++p) {
+        s1[((size_t)s * np + p) * 2u] = strat1[s][p][0];
+        s1[((size_t)s * np + p) * 2u + 1u] = strat1[s][p][1];
+      }
+    HIP_CALL(hipMemcpy(d_s1, s1.data(), sizeof(uint16_t) * s1.size(), hipMemcpyHostToDevice));
+    std::vector<uint8_t> s0(STRAT0_TAB_BYTES, 0u);
+    for (unsigned s = 0u; s < s0steps; ++s)
+      for (unsigned p = 0u; p < 16u; ++p) {
+        s0[(s * 16u + p) * 2u] = strat0[s][p][0];
+        s0[(s * 16u + p) * 2u + 1u] = strat0[s][p][1];
+      }
+    HIP_CALL(hipMemcpy(d_s0, s0.data(), STRAT0_TAB_BYTES, hipMemcpyHostToDevice));
+  }
+  void release()
+  {
+    HIP_CALL(hipFree(d_s1));
+    HIP_CALL(hipFree(d_s0));
+    HIP_CALL(hipFree(d_part));
+    HIP_CALL(hipFree(d_W));
+    HIP_CALL(hipFree(d_flags));
+    HIP_CALL(hipFree(d_pstat));
+    HIP_CALL(hipFree(d_cvg));
+    HIP_CALL(hipHostFree(h_cvg));
+  }
+};
+
+static void check_tables(unsigned ncol)
+{
+  if (STRAT0_PAIRS != 16u || !STRAT0_STEPS)
+    HJ_DIE("init_strats: level-0 strategy must have order 32");
+  if (2u * STRAT1_PAIRS != ncol / HYPJACL1_NCOLB || !STRAT1_STEPS)
+    HJ_DIE("init_strats: level-1 strategy order differs from ncol/16");
+}
+
+// One level-1 step: Gram partials, pair kernels, block update.
+static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, unsigned nswp,
+                       int definite, unsigned nplus, double *dG, unsigned lddG, unsigned nrow,
+                       double *dV, unsigned lddV, unsigned ncol, hipEvent_t *ev)
+{
+  const uint16_t *pairs = w.d_s1 + (size_t)step * w.np * 2u;
+  if (ev) HIP_CALL(hipEventRecord(ev[0], s));
+  launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part);
+  if (ev) HIP_CALL(hipEventRecord(ev[1], s));
+  launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_cvg, w.d_pstat,
+              w.d_s0, w.s0steps, nswp, definite, nplus);
+  if (ev) HIP_CALL(hipEventRecord(ev[2], s));
+  launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
+  if (ev) HIP_CALL(hipEventRecord(ev[3], s));
+}
+
+// Sweeps + initD on device data (the body of 1/HypJacL2.cu:104-178).
+static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
+                        unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
+                        unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t s)
+{
+  const bool cdsort = (routine & HYPJAC_CDSORT_1) != 0u;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  const bool blk_ori = (routine & HYPJAC_BLK_ORI) != 0u;
+  const unsigned nswp = blk_ori ? 1u : HYPJAC_NSWEEP;
+  check_tables(ncol);
+
+  Work w;
+  w.alloc(nrow, ncol, full_svd);
+  if (full_svd)
+    launch_initV(s, dV, 0u, ncol, lddV);
+  const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
+  double *const V = full_svd ? dV : nullptr;
+
+  g_stats = Stats();
+  *glb_s = 0ull;
+  *glb_b = 0ull;
+  const bool eager = g_profile || (g_step_limit > 0) || getenv("HJ_NOGRAPH");
+  hipGraphExec_t gexec = nullptr;
+  std::vector<hipEvent_t> evs;
+  if (g_profile) {
+    evs.resize((size_t)4u * w.steps);
+    for (auto &e : evs) HIP_CALL(hipEventCreate(&e));
+  }
+  if (!eager) {
+    // one sweep as a graph: zero the counter (a kernel, no memset node),
+    // all steps, copy the counter to pinned host memory
+    hipGraph_t graph;
+    HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_zero_u64(s, w.d_cvg);
+    for (unsigned st = 0u; st < w.steps; ++st)
+      issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr);
+    HIP_CALL(hipMemcpyAsync(w.h_cvg, w.d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_CALL(hipStreamEndCapture(s, &graph));
+    HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    HIP_CALL(hipGraphDestroy(graph));
+  }
+
+  long long swp_tim = 0;
+  stopwatch_reset(swp_tim);
+  unsigned blk_swp = 0u;
+  long steps_done = 0;
+  bool stop = false;
+  while (blk_swp < HYPJAC_NSWEEP) {
+    if (eager) {
+      launch_zero_u64(s, w.d_cvg);
+      for (unsigned st = 0u; st < w.steps; ++st) {
+        if (g_step_limit > 0 && steps_done >= g_step_limit) {
+          stop = true;
+          break;
+        }
+        ++steps_done;
+        issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
+                   g_profile ? &evs[(size_t)4u * st] : nullptr);
+      }
+      HIP_CALL(hipMemcpyAsync(w.h_cvg, w.d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    }
+    else
+      HIP_CALL(hipGraphLaunch(gexec, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    if (g_profile)
+      for (unsigned st = 0u; st < w.steps; ++st)
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0.0f;
+          HIP_CALL(hipEventElapsedTime(&ms, evs[(size_t)4u * st + k], evs[(size_t)4u * st + k + 1]));
+          g_stats.kms[k] += ms;
+          g_stats.kcnt[k] += 1ull;
+        }
+    ++blk_swp;
+    const unsigned long long cvg = *w.h_cvg;
+    const unsigned cvg_s = static_cast<unsigned>(cvg);
+    const unsigned cvg_b = static_cast<unsigned>(cvg >> 32u);
+    *glb_s += cvg_s;
+    *glb_b += cvg_b;
+    const double tim_s = stopwatch_lap(swp_tim) * TS2S;
+    g_stats.rot_s[blk_swp - 1u] = cvg_s;
+    g_stats.rot_b[blk_swp - 1u] = cvg_b;
+    g_stats.sec[blk_swp - 1u] = tim_s;
+    g_stats.nswp = blk_swp;
+    if (!quiet()) {
+      (void)fprintf(stdout, "BLK_SWP(%2u), ROT_S(%10u), ROT_B(%10u), TIME(%#12.6f s)\n", blk_swp,
+                    cvg_s, cvg_b, tim_s);
+      (void)fflush(stdout);
+    }
+    if (!cvg_b || stop)
+      break;
+  }
+  if (gexec) HIP_CALL(hipGraphExecDestroy(gexec));
+  for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
+
+  *glbSwp = blk_swp;
+  launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
+  HIP_CALL(hipStreamSynchronize(s));
+  w.release();
+}
+
+// argument checks of 1/HypJacL2.cu:39-77
+static int check_args(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
+                      const void *G, unsigned ldG, const void *V, unsigned ldV, const void *D,
+                      const void *glbSwp, const void *glb_s, const void *glb_b)
+{
+  if (routine >= 16u) return -1;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  if (!nrow || (nrow % 64u)) return -2;
+  if (!ncol || (ncol > nrow) || (ncol % 32u)) return -3;
+  if (nplus > ncol) return -4;
+  if (!G) return -5;
+  if (ldG < nrow) return -6;
+  if (full_svd) {
+    if (!V) return -7;
+    if (ldV < ncol) return -8;
+  }
+  if (!D) return -9;
+  if (!glbSwp) return -10;
+  if (!glb_s) return -11;
+  if (!glb_b) return -12;
+  return 0;
+}
+
+// Thin level-1 layer state (share/device_code_globals.hpp:4-20).
+struct Thin {
+  double *G = nullptr, *V = nullptr;
+  unsigned long long *cvg = nullptr;
+  unsigned nRow = 0u, nCol = 0u, ldG = 0u, ldV = 0u, nSwp = 0u;
+  Work w;
+  bool ready = false;
+};
+static Thin g_thin;
+
+static void thin_step(unsigned step, bool sort, bool withV, int definite, unsigned nplus)
+{
+  if (!g_thin.ready) HJ_DIE("initSymbols not called");
+  if (step >= g_thin.w.steps) HJ_DIE("step out of range");
+  Work w = g_thin.w;
+  w.d_cvg = g_thin.cvg;
+  issue_step(nullptr, w, step, sort, g_thin.nSwp, definite, nplus, g_thin.G, g_thin.ldG,
+             g_thin.nRow, withV ? g_thin.V : nullptr, g_thin.ldV, g_thin.nCol, nullptr);
+  HIP_CALL(hipGetLastError());
+}
+
+}  // namespace hj
+
+using namespace hj;
+
+extern "C" int hypJacL2(const unsigned routine, const unsigned nrow, const unsigned ncol,
+                        const unsigned nplus, double *const hG, const unsigned ldhG,
+                        double *const hV, const unsigned ldhV, double *const hD,
+                        unsigned *const glbSwp, unsigned long long *const glb_s,
+                        unsigned long long *const glb_b, double *const timing)
+{
+  long long timers[4] = {0ll, 0ll, 0ll, 0ll};
+  stopwatch_reset(timers[0]);
+  const int rc = check_args(routine, nrow, ncol, nplus, hG, ldhG, hV, ldhV, hD, glbSwp, glb_s, glb_b);
+  if (rc)
+    return rc;
+  stopwatch_reset(timers[3]);
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  const unsigned lddG = nrow, lddV = full_svd ? ncol : 0u;
+  double *dG = nullptr, *dV = nullptr, *dD = nullptr;
+  HIP_CALL(hipMalloc(&dG, sizeof(double) * (size_t)lddG * ncol));
+  if (full_svd)
+    HIP_CALL(hipMalloc(&dV, sizeof(double) * (size_t)lddV * ncol));
+  HIP_CALL(hipMalloc(&dD, sizeof(double) * ncol));
+  hipStream_t s = nullptr;
+  HIP_CALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_CALL(hipMemcpy2DAsync(dG, lddG * sizeof(double), hG, ldhG * sizeof(double),
+                            nrow * sizeof(double), ncol, hipMemcpyHostToDevice, s));
+  HIP_CALL(hipStreamSynchronize(s));
+  timers[1] = stopwatch_lap(timers[3]);
+
+  level2_core(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b, s);
+  timers[2] = stopwatch_lap(timers[3]);
+
+  HIP_CALL(hipMemcpy2DAsync(hG, ldhG * sizeof(double), dG, lddG * sizeof(double),
+                            nrow * sizeof(double), ncol, hipMemcpyDeviceToHost, s));
+  if (full_svd)
+    HIP_CALL(hipMemcpy2DAsync(hV, ldhV * sizeof(double), dV, lddV * sizeof(double),
+                              ncol * sizeof(double), ncol, hipMemcpyDeviceToHost, s));
+  HIP_CALL(hipMemcpyAsync(hD, dD, ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CALL(hipStreamSynchronize(s));
+  HIP_CALL(hipStreamDestroy(s));
+  HIP_CALL(hipFree(dD));
+  if (dV) HIP_CALL(hipFree(dV));
+  HIP_CALL(hipFree(dG));
+  timers[3] = stopwatch_lap(timers[3]);
+  timers[0] = stopwatch_lap(timers[0]);
+  if (timing)
+    for (unsigned i = 0u; i < 4u; ++i)
+      timing[i] = timers[i] * TS2S;
+  return 0;
+}
+
+extern "C" int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
+                            double *dG, unsigned lddG, double *dV, unsigned lddV, double *dD,
+                            unsigned *glbSwp, unsigned long long *glb_s,
+                            unsigned long long *glb_b, double *timing, void *stream)
+{
+  long long t0 = 0, t1 = 0;
+  stopwatch_reset(t0);
+  const int rc = check_args(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b);
+  if (rc)
+    return rc;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  if ((lddG & 1u) || (reinterpret_cast<uintptr_t>(dG) & 15u)) return -13;
+  if (full_svd && ((lddV & 1u) || (reinterpret_cast<uintptr_t>(dV) & 15u))) return -13;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  stopwatch_reset(t1);
+  level2_core(routine, nrow, ncol, nplus, dG, lddG, full_svd ? dV : nullptr, lddV, dD, glbSwp,
+              glb_s, glb_b, s);
+  const long long tc = stopwatch_lap(t1);
+  const long long tw = stopwatch_lap(t0);
+  if (timing) {
+    timing[0] = tw * TS2S;
+    timing[1] = 0.0;
+    timing[2] = tc * TS2S;
+    timing[3] = 0.0;
+  }
+  return 0;
+}
+
+extern "C" void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b, double *sec)
+{
+  if (nswp) *nswp = g_stats.nswp;
+  for (unsigned i = 0u; i < g_stats.nswp; ++i) {
+    if (rot_s) rot_s[i] = g_stats.rot_s[i];
+    if (rot_b) rot_b[i] = g_stats.rot_b[i];
+    if (sec) sec[i] = g_stats.sec[i];
+  }
+}
+
+extern "C" void hj_set_profile(int on) { g_profile = on; }
+
+extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
+{
+  if (kind < 0 || kind > 2) return -1.0;
+  if (launches) *launches = g_stats.kcnt[kind];
+  return g_stats.kms[kind];
+}
+
+extern "C" unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs) { return gram_nchunk(nrow, npairs); }
+
+extern "C" void hj_set_step_limit(long steps) { g_step_limit = steps; }
+
+extern "C" int hj_dbg_scalar(int kind, int n, const double *a, const double *b, const double *c,
+                             double *out_c, double *out_t, int *out_big)
+{
+  if (n <= 0) return 0;
+  double *da, *db, *dc, *oc, *ot;
+  int *ob;
+  const size_t B = sizeof(double) * (size_t)n;
+  HIP_CALL(hipMalloc(&da, B));
+  HIP_CALL(hipMalloc(&db, B));
+  HIP_CALL(hipMalloc(&dc, B));
+  HIP_CALL(hipMalloc(&oc, B));
+  HIP_CALL(hipMalloc(&ot, B));
+  HIP_CALL(hipMalloc(&ob, sizeof(int) * (size_t)n));
+  HIP_CALL(hipMemcpy(da, a, B, hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(db, b ? b : a, B, hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(dc, c ? c : a, B, hipMemcpyHostToDevice));
+  launch_dbg_scalar(nullptr, kind, n, da, db, dc, oc, ot, ob);
+  HIP_CALL(hipGetLastError());
+  HIP_CALL(hipDeviceSynchronize());
+  HIP_CALL(hipMemcpy(out_c, oc, B, hipMemcpyDeviceToHost));
+  HIP_CALL(hipMemcpy(out_t, ot, B, hipMemcpyDeviceToHost));
+  HIP_CALL(hipMemcpy(out_big, ob, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+  HIP_CALL(hipFree(da));
+  HIP_CALL(hipFree(db));
+  HIP_CALL(hipFree(dc));
+  HIP_CALL(hipFree(oc));
+  HIP_CALL(hipFree(ot));
+  HIP_CALL(hipFree(ob));
+  return 0;
+}
+
+extern "C" const char *hj_version(void) { return "gpujachx_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// Device configuration, share/cuda_helper.cu:5-54.
+
+extern "C" int configureGPUex(const int dev, const unsigned maxShMemB)
+{
+  if (dev < 0) HJ_DIE("device number negative");
+  (void)maxShMemB;  // LDS is 160 KiB per CU on gfx950; no cache split to choose
+  HIP_CALL(hipSetDevice(dev));
+  hipDeviceProp_t p;
+  HIP_CALL(hipGetDeviceProperties(&p, dev));
+  if (p.warpSize != 64) {
+    char msg[256];
+    (void)snprintf(msg, sizeof(msg), "HIP device %d has %d lanes per wave, must be 64", dev, p.warpSize);
+    HJ_DIE(msg);
+  }
+  if (strncmp(p.gcnArchName, "gfx9", 4)) {
+    char msg[256];
+    (void)snprintf(msg, sizeof(msg), "HIP device %d is %s, not a gfx9 (CDNA) part", dev, p.gcnArchName);
+    HJ_DIE(msg);
+  }
+  return p.major * 10 + p.minor;
+}
+
+extern "C" int configureGPU(const int dev) { return configureGPUex(dev, 16384u); }
+
+// ---------------------------------------------------------------------------
+// Thin level-1 layer, share/device_code.cu:15-118.
+
+extern "C" void initSymbols(double *const G, double *const V, volatile unsigned long long *const cvg,
+                            const unsigned nRow, const unsigned nCol, const unsigned ldG,
+                            const unsigned ldV, const unsigned nSwp)
+{
+  if (g_thin.ready) {
+    g_thin.w.release();
+    g_thin.ready = false;
+  }
+  check_tables(nCol);
+  g_thin.G = G;
+  g_thin.V = V;
+  g_thin.cvg = const_cast<unsigned long long *>(cvg);
+  g_thin.nRow = nRow;
+  g_thin.nCol = nCol;
+  g_thin.ldG = ldG;
+  g_thin.ldV = ldV;
+  g_thin.nSwp = nSwp;
+  g_thin.w.alloc(nRow, nCol, V != nullptr);
+  g_thin.ready = true;
+}
+
+extern "C" void defJacL1(const unsigned step, const int definite) { thin_step(step, false, false, definite ? definite : 1, 0u); }
+extern "C" void hypJacL1(const unsigned step, const unsigned npos) { thin_step(step, false, false, 0, npos); }
+extern "C" void defJacL1v(const unsigned step, const int definite) { thin_step(step, false, true, definite ? definite : 1, 0u); }
+extern "C" void hypJacL1v(const unsigned step, const unsigned npos) { thin_step(step, false, true, 0, npos); }
+extern "C" void defJacL1s(const unsigned step, const int definite) { thin_step(step, true, false, definite ? definite : 1, 0u); }
+extern "C" void hypJacL1s(const unsigned step, const unsigned npos) { thin_step(step, true, false, 0, npos); }
+extern "C" void defJacL1sv(const unsigned step, const int definite) { thin_step(step, true, true, definite ? definite : 1, 0u); }
+extern "C" void hypJacL1sv(const unsigned step, const unsigned npos) { thin_step(step, true, true, 0, npos); }
+
+extern "C" void initD(double *const G, double *const D, const unsigned ifc, const unsigned nRow,
+                      const unsigned nRank, const unsigned nPlus, const unsigned ldG)
+{
+  launch_initD(nullptr, G, D, ifc, nRow, nRank, nPlus, ldG);
+  HIP_CALL(hipGetLastError());
+}
+
+extern "C" void initV(double *const V, const unsigned ifc, const unsigned nRank, const unsigned ldV)
+{
+  launch_initV(nullptr, V, ifc, nRank, ldV);
+  HIP_CALL(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Built-in strategy tables (no strat.so needed).
+
+extern "C" int hj_strat_generate(const char *fam, unsigned n, unsigned short *out, unsigned *steps)
+{
+  Strategy st;
+  if (!generate(fam, n, st))
+    return -1;
+  if (steps) *steps = st.steps;
+  if (out) memcpy(out, st.tab.data(), sizeof(uint16_t) * st.tab.size());
+  return 0;
+}
+
+extern "C" void hj_init_strats_builtin(const char *snp0, unsigned n0, const char *snp1, unsigned n1)
+{
+  Strategy a, b;
+  if (!snp0 || !snp1) HJ_DIE("strategy name empty");
+  if (n0 != 2u * STRAT0_MAX_PAIRS) HJ_DIE("n0 must be 32");
+  if (!n1 || (n1 & 1u) || n1 > 2u * STRAT1_MAX_PAIRS) HJ_DIE("n1 not positive even <= 1024");
+  if (!generate(snp0, n0, a)) HJ_DIE("NO SUCH STRATEGY for SNP0 & n0");
+  if (!generate(snp1, n1, b)) HJ_DIE("NO SUCH STRATEGY for SNP1 & n1");
+  STRAT0 = family_id(snp0);
+  STRAT0_STEPS = a.steps;
+  STRAT0_PAIRS = a.pairs;
+  memset(strat0, 0, sizeof(strat0));
+  for (unsigned s = 0u; s < a.steps; ++s)
+    for (unsigned p = 0u; p < a.pairs; ++p)
+      for (unsigned i = 0u; i < 2u; ++i)
+        strat0[s][p][i] = (unsigned char)a.tab[((size_t)s * a.pairs + p) * 2u + i];
+  STRAT1 = family_id(snp1);
+  STRAT1_STEPS = b.steps;
+  STRAT1_PAIRS = b.pairs;
+  memset(strat1, 0, sizeof(strat1));
+  for (unsigned s = 0u; s < b.steps; ++s)
+    for (unsigned p = 0u; p < b.pairs; ++p)
+      for (unsigned i = 0u; i < 2u; ++i)
+        strat1[s][p][i] = b.tab[((size_t)s * b.pairs + p) * 2u + i];
+}

@@ -1,0 +1,163 @@
+/* hypjac.h -- C ABI of the MI355X (gfx950) blocked one-sided (hyperbolic)
+ * Jacobi SVD library (libhypjac.so).
+ *
+ * The first block mirrors the entry points a GPUJACHx caller binds; each one
+ * names the reference declaration it replaces.  Semantics (argument codes,
+ * ALG bits, the stdout lines, fail-stop on runtime errors) are the same.
+ * The second block holds additions of this library (device-resident inputs,
+ * explicit tables, statistics, the split-precision and learned-skip forms,
+ * the batched launch).  No torch or HIP types appear in any signature.
+ */
+#ifndef HYPJAC_H
+#define HYPJAC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Reference surface                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* ALG bits (share/HypJacL.hpp:88-110) */
+#define HYPJAC_BLK_ORI  8u  /* one inner sweep per block pair (else <= 30) */
+#define HYPJAC_ACCUMV_2 4u  /* defined by the reference, never read */
+#define HYPJAC_FULL_SVD 2u  /* accumulate V */
+#define HYPJAC_CDSORT_1 1u  /* sort inside the 32-column blocks */
+
+/* strategy IDs (share/HypJacL.hpp:118-152) */
+#define STRAT_MMSTEP 1u
+#define STRAT_BRENTL 2u
+#define STRAT_COLCYC 3u
+#define STRAT_CYCLOC 4u
+#define STRAT_ROWCYC 5u
+#define STRAT_CYCWOR 6u
+#define STRAT_BLKREC 7u
+
+#define HYPJACL1_NCOLB   16u   /* columns per block (share/HypJacL.hpp:112-116) */
+#define HYPJAC_NSWEEP    30u   /* sweep cap per level (share/HypJacL.hpp:7-9) */
+#define STRAT0_MAX_PAIRS 16u
+#define STRAT0_MAX_STEPS 32u
+#define STRAT1_MAX_PAIRS 512u
+#define STRAT1_MAX_STEPS 1024u
+#define HYPJAC_IDADIM_SIZE 4u  /* share/hdf5_helper.hpp:9-13 */
+
+/* Level-2 driver.  Replaces hypJacL2 of 1/HypJacL2.hpp:6-27 (definition
+ * 1/HypJacL2.cu:13-202): same arguments, same return codes
+ * (-1 routine >= 16, -2 nrow, -3 ncol, -4 nplus, -5 hG, -6 ldhG, -7 hV,
+ * -8 ldhV, -9 hD, -10..-12 counters), host buffers in, host buffers out,
+ * device memory allocated and freed inside.  hG returns U (unit columns),
+ * hD = j_i * sigma_i^2, hV (if routine & 2) the accumulated V.  Requires a
+ * prior init_strats().  Prints one BLK_SWP line per sweep to stdout. */
+int hypJacL2(const unsigned routine, const unsigned nrow, const unsigned ncol,
+             const unsigned nplus, double *const hG, const unsigned ldhG,
+             double *const hV, const unsigned ldhV, double *const hD,
+             unsigned *const glbSwp, unsigned long long *const glb_s,
+             unsigned long long *const glb_b, double *const timing);
+
+/* Strategy loader.  Replaces init_strats of share/HypJacL.hpp:206
+ * (definition share/HypJacL.cu:11-121): dlopen(sdy), dlsym("<snp>%05u"),
+ * copy into the process-wide tables below; fail-stop (message + exit) on
+ * any error, as the reference does. */
+void init_strats(const char *const sdy, const char *const snp0,
+                 const unsigned n0, const char *const snp1, const unsigned n1);
+
+/* Process-wide strategy state, share/HypJacL.hpp:200-204. */
+extern unsigned STRAT0, STRAT0_STEPS, STRAT0_PAIRS;
+extern unsigned STRAT1, STRAT1_STEPS, STRAT1_PAIRS;
+extern unsigned char strat0[STRAT0_MAX_STEPS][STRAT0_MAX_PAIRS][2];
+extern unsigned short strat1[STRAT1_MAX_STEPS][STRAT1_MAX_PAIRS][2];
+
+/* Device selection.  Replaces configureGPU/configureGPUex of
+ * share/cuda_helper.hpp:29-30 (share/cuda_helper.cu:5-54).  Returns
+ * 10*major+minor of the device (95 for gfx950); dies if the device is not a
+ * 64-lane gfx9 part. */
+int configureGPU(const int dev);
+int configureGPUex(const int dev, const unsigned maxShMemB);
+
+/* Strategy plugin helpers, share/my_utils.hpp:30-33 (share/my_utils.cu:30-61). */
+void *strat_open(const char *const sdy);
+int strat_close(void *const h);
+const void *strat_ptr(void *const h, const char *const snp, const unsigned n);
+int fexist(const char *const fn);
+
+/* Thin level-1 layer on a process-wide context, share/device_code.hpp:6-21
+ * (share/device_code.cu:15-118).  initSymbols binds device G/V, the
+ * convergence counter (an 8-byte word in device memory, not host-mapped
+ * memory) and the sizes; each *JacL1* call launches one level-1 step
+ * (Gram, pair, update) for all pairs of STRAT1 step `step` on the default
+ * stream. */
+void initSymbols(double *const G, double *const V,
+                 volatile unsigned long long *const cvg, const unsigned nRow,
+                 const unsigned nCol, const unsigned ldG, const unsigned ldV,
+                 const unsigned nSwp);
+void defJacL1(const unsigned step, const int definite);
+void hypJacL1(const unsigned step, const unsigned npos);
+void defJacL1v(const unsigned step, const int definite);
+void hypJacL1v(const unsigned step, const unsigned npos);
+void defJacL1s(const unsigned step, const int definite);
+void hypJacL1s(const unsigned step, const unsigned npos);
+void defJacL1sv(const unsigned step, const int definite);
+void hypJacL1sv(const unsigned step, const unsigned npos);
+void initD(double *const G, double *const D, const unsigned ifc,
+           const unsigned nRow, const unsigned nRank, const unsigned nPlus,
+           const unsigned ldG);
+void initV(double *const V, const unsigned ifc, const unsigned nRank,
+           const unsigned ldV);
+
+/* ---------------------------------------------------------------------- */
+/* Additions of this library                                                */
+/* ---------------------------------------------------------------------- */
+
+/* hypJacL2 on device-resident data: dG (lddG x ncol), dV (lddV x ncol, if
+ * routine & 2), dD (ncol) are device pointers; lddG, lddV must be even and
+ * the pointers 16-byte aligned.  Work is issued on `stream` (a hipStream_t,
+ * NULL = default stream); returns after the result is complete.  Same
+ * return codes as hypJacL2 (-5/-7/-9 for NULL pointers, -13 alignment). */
+int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol,
+                 unsigned nplus, double *dG, unsigned lddG, double *dV,
+                 unsigned lddV, double *dD, unsigned *glbSwp,
+                 unsigned long long *glb_s, unsigned long long *glb_b,
+                 double *timing, void *stream);
+
+/* Per-sweep statistics of the last hypJacL2/hypJacL2_dev call in this
+ * process: rot_s[i], rot_b[i] for sweeps i < *nswp (arrays of >= 30). */
+void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
+                    double *sweep_seconds);
+
+/* Kernel timing of the last call (HIP events on the work stream, enabled
+ * by hj_set_profile(1); graphs are not used while profiling):
+ * kind 0 = Gram, 1 = pair, 2 = update; returns total ms and launch count. */
+void hj_set_profile(int on);
+double hj_kernel_ms(int kind, unsigned long long *launches);
+
+/* Number of row chunks of the Gram of one block pair (for nrow rows and
+ * npairs pairs per step); the oracle reproduces the same split. */
+unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs);
+
+/* Test limit: stop after `steps` level-1 steps in total (0 = no limit). */
+void hj_set_step_limit(long steps);
+
+/* Device self-test of the scalar kernels: for n inputs, kind 0 = rsqrt(a),
+ * 1 = trigonometric rotation (a=Apq,b=Dp,c=Dq), 2 = hyperbolic rotation.
+ * Outputs c, t, big per input.  Returns 0 on success. */
+int hj_dbg_scalar(int kind, int n, const double *a, const double *b,
+                  const double *c, double *out_c, double *out_t, int *out_big);
+
+/* Built-in strategy tables: generate family `fam` of order n into
+ * out[steps][n/2][2] (may be NULL to query *steps); -1 if this library has
+ * no construction for (fam, n).  hj_init_strats_builtin fills the same
+ * process-wide tables as init_strats without a strat.so. */
+int hj_strat_generate(const char *fam, unsigned n, unsigned short *out,
+                      unsigned *steps);
+void hj_init_strats_builtin(const char *snp0, unsigned n0, const char *snp1,
+                            unsigned n1);
+
+/* Version string of the library build. */
+const char *hj_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYPJAC_H */

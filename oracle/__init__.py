@@ -1,0 +1,125 @@
+"""CPU FP64 oracle of the blocked one-sided (hyperbolic) Jacobi SVD.
+
+TEST INFRASTRUCTURE ONLY: the checker of the gfx950 library.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.  It
+wraps oracle/libhjoracle.so (hjoracle.c: a C restatement of the reference's
+algorithm, 1/HypJacL2.cu and share/device_code_*.hpp, in the device's exact
+operation order).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(_HERE, "libhjoracle.so")
+REF_STRAT_SO = os.path.join(_HERE, "_ref", "strat.so")
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(SO):
+            raise RuntimeError("oracle/libhjoracle.so missing: run `python -m gpujachx_amd.build`")
+        L = ctypes.CDLL(SO)
+        d, u, i = ctypes.c_double, ctypes.c_uint, ctypes.c_int
+        pd, pu, pi = ctypes.POINTER(d), ctypes.POINTER(u), ctypes.POINTER(i)
+        pull = ctypes.POINTER(ctypes.c_ulonglong)
+        L.hjo_rsqrt.argtypes = [d]
+        L.hjo_rsqrt.restype = d
+        L.hjo_rot_t.argtypes = [d, d, d, pd, pd]
+        L.hjo_rot_t.restype = i
+        L.hjo_rot_h.argtypes = [d, d, d, pd, pd]
+        L.hjo_rot_h.restype = i
+        L.hjo_sum32.argtypes = [pd]
+        L.hjo_sum32.restype = d
+        L.hjo_gram_nchunk.argtypes = [u, u]
+        L.hjo_gram_nchunk.restype = u
+        L.hjo_gram_partial.argtypes = [pd, pd, u, u, u, pd]
+        L.hjo_gram_partial.restype = None
+        L.hjo_pair.argtypes = [pd, u, i, i, i, u, ctypes.c_void_p, u, u, pd, pu, pi, pd]
+        L.hjo_pair.restype = u
+        L.hjo_update.argtypes = [pd, pd, u, u, pd]
+        L.hjo_update.restype = None
+        L.hjo_initD.argtypes = [pd, u, u, u, u, pd]
+        L.hjo_initD.restype = None
+        L.hjo_hypJacL2.argtypes = [u, u, u, u, pd, u, pd, u, pd, pu, pull, pull,
+                                   ctypes.c_void_p, u, ctypes.c_void_p, u, i, ctypes.c_long, pu, pu]
+        L.hjo_hypJacL2.restype = i
+        _L = L
+    return _L
+
+
+def _pd(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def rsqrt(x):
+    return lib().hjo_rsqrt(float(x))
+
+
+def rot_t(apq, dp, dq):
+    c, t = ctypes.c_double(), ctypes.c_double()
+    b = lib().hjo_rot_t(apq, dp, dq, ctypes.byref(c), ctypes.byref(t))
+    return c.value, t.value, b
+
+
+def rot_h(apq, dp, dq):
+    c, t = ctypes.c_double(), ctypes.c_double()
+    b = lib().hjo_rot_h(apq, dp, dq, ctypes.byref(c), ctypes.byref(t))
+    return c.value, t.value, b
+
+
+def sum32(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    assert v.size == 32
+    return lib().hjo_sum32(_pd(v))
+
+
+def gram_nchunk(nrow, npairs):
+    return lib().hjo_gram_nchunk(nrow, npairs)
+
+
+def hypJacL2(routine, G, nplus, strat0, strat1, ldhG=None, nthreads=0, max_steps=0):
+    """Mirror of 1/HypJacL2.cu:13-202 on numpy data (G: nrow x ncol).
+    strat0: uint8 [S0,16,2]; strat1: uint16 [S1,n1/2,2].
+    Returns dict(ret, G(=U), V, D, sweeps, rot_s, rot_b, rot_s_trace, rot_b_trace)."""
+    G = np.asarray(G, dtype=np.float64)
+    nrow, ncol = G.shape
+    ld = nrow if ldhG is None else ldhG
+    hG = np.zeros((ncol, ld), dtype=np.float64)
+    hG[:, :nrow] = G.T
+    full = bool(routine & 2)
+    hV = np.zeros((ncol, ncol), dtype=np.float64) if full else None
+    hD = np.zeros(ncol, dtype=np.float64)
+    s0 = np.ascontiguousarray(strat0, dtype=np.uint8)
+    s1 = np.ascontiguousarray(strat1, dtype=np.uint16)
+    sw = ctypes.c_uint(0)
+    gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    ts = (ctypes.c_uint * 30)()
+    tb = (ctypes.c_uint * 30)()
+    ret = lib().hjo_hypJacL2(routine, nrow, ncol, nplus, _pd(hG), ld, _pd(hV), ncol if full else 0,
+                             _pd(hD), ctypes.byref(sw), ctypes.byref(gs), ctypes.byref(gb),
+                             s0.ctypes.data, s0.shape[0], s1.ctypes.data, s1.shape[0],
+                             int(nthreads), int(max_steps), ts, tb)
+    k = sw.value
+    return dict(ret=ret, G=hG[:, :nrow].T.copy(), V=None if hV is None else hV.T.copy(), D=hD,
+                sweeps=k, rot_s=gs.value, rot_b=gb.value,
+                rot_s_trace=list(ts[:k]), rot_b_trace=list(tb[:k]))
+
+
+def pair(partials, sort, hyp, definite, npos, strat0, nswp=1):
+    """One block pair from its Gram partials; returns (W, R, s, b, upd)."""
+    p = np.ascontiguousarray(partials, dtype=np.float64).reshape(-1)
+    nchunk = p.size // 768
+    s0 = np.ascontiguousarray(strat0, dtype=np.uint8)
+    W = np.zeros(1024)
+    R = np.zeros(1024)
+    b = ctypes.c_uint(0)
+    upd = ctypes.c_int(0)
+    s = lib().hjo_pair(_pd(p), nchunk, int(sort), int(hyp), int(definite), int(npos),
+                       s0.ctypes.data, s0.shape[0], nswp, _pd(W), ctypes.byref(b), ctypes.byref(upd),
+                       _pd(R))
+    return W.reshape(32, 32).T.copy(), R.reshape(32, 32).T.copy(), s, b.value, upd.value

@@ -1,0 +1,108 @@
+/* hjoracle.h -- CPU FP64 restatement of the hierarchically blocked one-sided
+ * (hyperbolic) Jacobi SVD of venovako/GPUJACHx (level-2 driver 1/HypJacL2.cu,
+ * level-1 kernels share/device_code_{nosort,cdsort}_{solveV,accumV}.hpp).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (libhypjac.so) never links or calls it.
+ *
+ * Arithmetic order.  Every floating-point operation is an IEEE-754 double
+ * operation (fma, +, *, /, sqrt) in the exact order the gfx950 kernels use, so
+ * the GPU path is expected to match this oracle bit for bit:
+ *   - Gram partial of a row chunk: 64-row groups ascending; inside a group,
+ *     16 steps t, each a chain over k=0..3 of fma(x,y,acc) on row
+ *     64g+16k+t (the gfx950 FP64 MFMA 16x16x4 is a sequential fma chain in k,
+ *     measured on the device);  partials of the chunks are added in order.
+ *   - Block update: out(r,j) = fma chain over k=0..31 of G(r,k)*W(k,j) from +0.
+ *   - Column norms in initD: 64 lane-strided fma chains + xor butterfly over
+ *     64 lanes (32,16,8,4,2,1).
+ * Everything else (Cholesky, inner Jacobi, rotations, sort/swap rules, the
+ * 32-lane xor butterfly 16,8,4,2,1 of the dot products, convergence counting)
+ * follows the reference's own order.  The only departure from the reference
+ * is the summation order inside the Gram and the update (the reference's is
+ * a skewed per-thread chain, share/device_code_common_Cholesky.hpp:31-40 and
+ * share/device_code_common.hpp:166-178); both are exact in real arithmetic.
+ */
+#ifndef HJORACLE_H
+#define HJORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* constants of the method (share/HypJacL.hpp:11-51) */
+#define HJO_SQRT_HEPS   0x1.6a09e667f3bcdp-27 /* sqrt(2^-53) */
+#define HJO_SQRT_2_HEPS 0x1p27                /* sqrt(2/2^-53) */
+#define HJO_MYTOL       0x1.6a09e667f3bcdp-51 /* sqrt(32)*2^-53 */
+#define HJO_COSHI_FIX   0x1.279a74590331cp+0  /* 2/sqrt(3), correctly rounded */
+#define HJO_TANH1_FIX   0.5
+#define HJO_NSWEEP      30u
+
+/* ALG bits (share/HypJacL.hpp:88-110) */
+#define HJO_BLK_ORI  8u
+#define HJO_FULL_SVD 2u
+#define HJO_CDSORT_1 1u
+
+/* correctly rounded 1/sqrt(x) for positive normal x (own method: IEEE
+ * 1/sqrt(x), then a neighbour test with a double-double residual). */
+double hjo_rsqrt(double x);
+
+/* 2x2 rotations, reference share/device_code_common_rotate.hpp:15-64
+ * (USE_RSQRT branch).  Return 1 for a "big" rotation. */
+int hjo_rot_t(double Apq, double Dp, double Dq, double *c, double *t);
+int hjo_rot_h(double Apq, double Dp, double Dq, double *c, double *t);
+
+/* sum of 32 lane values by the xor butterfly 16,8,4,2,1
+ * (share/device_code_common_Kepler.hpp:26-74) */
+double hjo_sum32(const double v[32]);
+
+/* number of 64-row-aligned row chunks the Gram of one block pair is split
+ * into; identical to the device library's choice for (nrow, pairs). */
+unsigned hjo_gram_nchunk(unsigned nrow, unsigned npairs);
+
+/* Gram partial of rows [r0, r0+nr) (nr % 64 == 0) of the column pair
+ * (G0: 16 columns of block i, G1: 16 columns of block j, leading dim ld).
+ * out[768]: tiles (00,10,11) x lane(64) x reg(4) in the MFMA D layout. */
+void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
+                      unsigned r0, unsigned nr, double out[768]);
+
+/* one block pair: sum nchunk partials, Cholesky, inner Jacobi (level 0),
+ * producing the 32x32 column-major transformation W.
+ * kind: 0 = definite (definite = +1/-1), 1 = hyperbolic (npos local).
+ * Returns the block's rotation count; *big = big rotations;
+ * *upd = 1 if the block update must be applied. */
+unsigned hjo_pair(const double *partials, unsigned nchunk, int sort,
+                  int hyp, int definite, unsigned npos,
+                  const uint8_t *strat0, unsigned s0steps, unsigned nswp,
+                  double W[1024], unsigned *big, int *upd, double R[1024]);
+
+/* [G0 G1](rows 0..nr-1) <- [G0 G1] * W  (device order) */
+void hjo_update(double *G0, double *G1, unsigned ld, unsigned nr,
+                const double W[1024]);
+
+/* column normalisation: G <- U, D = +-||g||^2 (device order) */
+void hjo_initD(double *G, unsigned ld, unsigned nrow, unsigned ncol,
+               unsigned nplus, double *D);
+
+/* Whole level-2 driver, reference 1/HypJacL2.cu:13-202 (returns the same
+ * argument codes).  strat0: [s0steps][16][2] uint8, strat1: [s1steps][n1/2][2]
+ * uint16.  nthreads <= 0: all OpenMP threads.  max_steps > 0 stops after
+ * that many level-1 steps in total (bounded CPU-timing sample; the result is
+ * then not converged).  rot_s_trace/rot_b_trace (optional, length >= 30)
+ * receive the per-sweep ROT_S / ROT_B. */
+int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
+                 double *G, unsigned ldG, double *V, unsigned ldV, double *D,
+                 unsigned *glbSwp, unsigned long long *glb_s,
+                 unsigned long long *glb_b,
+                 const uint8_t *strat0, unsigned s0steps,
+                 const uint16_t *strat1, unsigned s1steps,
+                 int nthreads, long max_steps,
+                 unsigned *rot_s_trace, unsigned *rot_b_trace);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HJORACLE_H */
