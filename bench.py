@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Headline benchmark: seconds per SVD of an M=N=8192 FP64 matrix on one MI355X.
+
+Workload (the split-precision configuration's matrix): G = U diag(s) V^T with
+Haar U, V (QR of a seeded N(0,1) matrix, column signs from diag R) and
+s_i = 10^(-12 u_i), u_i ~ U(0,1); strategies cycwor/cycwor; ALG 9 (block
+oriented, sorted, sigma only).  A step is one complete SVD (all sweeps until
+no big rotation, then the column normalisation) of a fresh copy of G that is
+already resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--alg A] [--no-cpu]
+
+Prints one JSON line.  One GPU only: --gpus above 1 exits with a message.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s per HSVD at N=8192 (learned+split vs plain FP64, MI355X), rel sigma err<=1e-12"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def make_input(torch, n, seed):
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    dev = torch.device("cuda")
+
+    def haar():
+        a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=gen)
+        q, r = torch.linalg.qr(a)
+        return q * torch.sign(torch.diagonal(r))
+
+    U = haar()
+    V = haar()
+    u = torch.rand(n, dtype=torch.float64, device=dev, generator=gen)
+    s = torch.pow(10.0, -12.0 * u)
+    G = (U * s) @ V.T
+    # column-major G (ld = n) is the row-major transpose
+    return G.T.contiguous(), s
+
+
+def cpu_baseline(G_host, n, alg, sweeps, steps_per_sweep, budget_s=15.0):
+    """The oracle (C restatement, OpenMP over the pairs of a step) timed on a
+    bounded sample: the first S level-1 steps of the same SVD; the per-step
+    time is scaled to the GPU run's step count."""
+    import oracle as o
+    import gpujachx_amd as g
+    s0, s1 = g.current_strats()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = min(threads, 16)
+    t0 = time.time()
+    o.hypJacL2(alg, G_host, n, s0, s1, nthreads=threads, max_steps=1)
+    t1 = time.time() - t0
+    S = max(1, min(steps_per_sweep, int(budget_s / max(t1, 1e-3))))
+    t0 = time.time()
+    o.hypJacL2(alg, G_host, n, s0, s1, nthreads=threads, max_steps=S)
+    tS = time.time() - t0
+    per_step = tS / S
+    total_steps = sweeps * steps_per_sweep
+    return {
+        "value": per_step * total_steps, "unit": "s", "cores": threads, "kind": "port",
+        "sample": "oracle (C, FP64, OpenMP over the pairs of a step) on the first %d of the %d "
+                  "level-1 steps of the same %dx%d SVD (%.2f s), scaled by %d sweeps x %d steps"
+                  % (S, total_steps, n, n, tS, sweeps, steps_per_sweep),
+    }
+
+
+def dgesvj_sample(n_small=1024, seed=3):
+    """LAPACK dgesvj on the same construction at a smaller order, for scale."""
+    from oracle.lapack import dgesvj
+    rng = np.random.default_rng(seed)
+
+    def haar(k):
+        q, r = np.linalg.qr(rng.standard_normal((k, k)))
+        return q * np.sign(np.diag(r))
+
+    s = 10.0 ** (-12.0 * rng.uniform(0, 1, n_small))
+    G = (haar(n_small) * s) @ haar(n_small).T
+    t0 = time.time()
+    _, _, _, info, stat = dgesvj(G, want_v=False)
+    return {"n": n_small, "seconds": time.time() - t0, "sweeps": int(stat[3]), "info": int(info)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--alg", type=int, default=9)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dgesvj", action="store_true", help="also time LAPACK dgesvj at N=1024")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU level "
+              "(the reference's is not in its repository); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    os.environ.setdefault("HJ_QUIET", "1")
+    import torch
+    import gpujachx_amd as g
+
+    n, alg = args.n, args.alg
+    g.configureGPU(0)
+    g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+    L = g.lib()
+    G0, s_exact = make_input(torch, n, args.seed)
+    W = torch.empty_like(G0)
+    D = torch.empty(n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sw = ctypes.c_uint(0)
+    gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    tm = (ctypes.c_double * 4)()
+
+    def one():
+        W.copy_(G0)
+        rc = L.hypJacL2_dev(alg, n, n, n, W.data_ptr(), n, None, 0, D.data_ptr(), ctypes.byref(sw),
+                            ctypes.byref(gs), ctypes.byref(gb), tm, stream.cuda_stream)
+        if rc:
+            raise RuntimeError("hypJacL2_dev returned %d" % rc)
+
+    for _ in range(args.warmup):
+        one()
+    # timed region: kernel durations by HIP events on the work stream
+    g.set_profile(True)
+    kms = np.zeros(3)
+    kcnt = np.zeros(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+        for k in range(3):
+            ms, c = g.kernel_ms(k)
+            kms[k] += ms
+            kcnt[k] += c
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.set_profile(False)
+    per = elapsed / args.steps
+    sweeps = sw.value
+    rot_s, rot_b, _ = g.last_sweeps()
+
+    # accuracy against the constructed spectrum
+    sig = torch.sqrt(torch.abs(D)).cpu().numpy()
+    ex = s_exact.cpu().numpy()
+    sig_s, ex_s = np.sort(sig)[::-1], np.sort(ex)[::-1]
+    rel = np.abs(sig_s - ex_s) / ex_s
+    normwise = np.max(np.abs(sig_s - ex_s)) / ex_s[0]
+
+    # roofline of the dominant kernel (the block update) and of the Gram
+    np_ = n // 32
+    steps_per_sweep = n // 16 - 1
+    upd_bytes = np_ * 512.0 * n          # read + write of the m x 32 pair, per launch
+    gram_bytes = np_ * 256.0 * n         # read of the m x 32 pair, per launch
+    upd_ms = kms[2] / max(kcnt[2], 1)
+    gram_ms = kms[0] / max(kcnt[0], 1)
+    pair_ms = kms[1] / max(kcnt[1], 1)
+    ach = upd_bytes / (upd_ms * 1e-3) / 1e9
+    gram_ach = gram_bytes / (gram_ms * 1e-3) / 1e9
+    upd_tf = np_ * 2048.0 * n / (upd_ms * 1e-3) / 1e12
+    gram_tf = np_ * 1536.0 * n / (gram_ms * 1e-3) / 1e12
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "update_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": per,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "SVD M=N=%d, G=U diag(10^-12u) V^T (Haar), ALG %d, cycwor/cycwor, "
+                               "plain FP64 form" % (n, alg),
+                   "M": n, "N": n, "alg": alg, "form": "plain-fp64", "seed": args.seed},
+        "sweeps": sweeps,
+        "rot_b_per_sweep": rot_b,
+        "sigma_err": {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
+                      "normwise": float(normwise),
+                      "frac_rel_le_1e-12": float(np.mean(rel <= 1e-12))},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_update", "avg_launch_ms": upd_ms,
+                     "algorithmic_bytes_per_launch": upd_bytes,
+                     "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},
+        "kernels": {"gram": {"avg_ms": gram_ms, "GBs": gram_ach, "tflops": gram_tf},
+                    "pair": {"avg_ms": pair_ms},
+                    "update": {"avg_ms": upd_ms, "GBs": ach, "tflops": upd_tf},
+                    "launches_per_step": float(kcnt[2] / args.steps)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(G0.T.cpu().numpy().copy(), n, alg, sweeps,
+                                           steps_per_sweep)
+    if args.dgesvj:
+        out["dgesvj_sample"] = dgesvj_sample()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

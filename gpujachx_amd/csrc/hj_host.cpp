@@ -136,8 +136,19 @@ static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, u
 // Sweeps + initD on device data (the body of 1/HypJacL2.cu:104-178).
 static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
                         unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
-                        unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t s)
+                        unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t caller)
 {
+  // Work runs on a stream of its own (capturable even when the caller uses
+  // the null stream), ordered after the caller's prior work.
+  hipStream_t s = nullptr;
+  HIP_CALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  {
+    hipEvent_t e;
+    HIP_CALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_CALL(hipEventRecord(e, caller));
+    HIP_CALL(hipStreamWaitEvent(s, e, 0));
+    HIP_CALL(hipEventDestroy(e));
+  }
   const bool cdsort = (routine & HYPJAC_CDSORT_1) != 0u;
   const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
   const bool blk_ori = (routine & HYPJAC_BLK_ORI) != 0u;
@@ -230,6 +241,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   *glbSwp = blk_swp;
   launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
   HIP_CALL(hipStreamSynchronize(s));
+  HIP_CALL(hipStreamDestroy(s));
   w.release();
 }
 

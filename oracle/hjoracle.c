@@ -426,6 +426,15 @@ static void local_npos(unsigned nplus, unsigned iBlk, unsigned jBlk,
   else { *npos = 32u; *definite = +1; }
 }
 
+/* optional per-visit trace: [sweep][step][pair] -> (rotations, big rotations) */
+static unsigned *g_trace = NULL;
+static size_t g_trace_cap = 0;
+void hjo_set_trace(unsigned *buf, size_t cap_entries)
+{
+  g_trace = buf;
+  g_trace_cap = cap_entries;
+}
+
 /* 1/HypJacL2.cu:13-202 */
 int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
                  double *G, unsigned ldG, double *V, unsigned ldV, double *D,
@@ -513,6 +522,14 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
           if (full_svd)
             hjo_update(V + (size_t)iBlk * 16u * ldV, V + (size_t)jBlk * 16u * ldV, ldV, ncol, W);
         }
+      }
+      if (g_trace) {
+        const size_t base = ((size_t)blk_swp * s1steps + step) * np;
+        for (unsigned pi = 0u; pi < np; ++pi)
+          if (base + pi < g_trace_cap) {
+            g_trace[2u * (base + pi)] = ps[2u * pi];
+            g_trace[2u * (base + pi) + 1u] = ps[2u * pi + 1u];
+          }
       }
       /* counter semantics of share/device_code_nosort.hpp:94-103 */
       for (unsigned pi = 0u; pi < np; ++pi) {

@@ -26,6 +26,7 @@
 #ifndef HJORACLE_H
 #define HJORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -100,6 +101,12 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
                  const uint16_t *strat1, unsigned s1steps,
                  int nthreads, long max_steps,
                  unsigned *rot_s_trace, unsigned *rot_b_trace);
+
+/* Per-visit trace for the following hjo_hypJacL2 calls: entry
+ * (sweep*s1steps + step)*npairs + pair holds (rotations, big rotations) at
+ * buf[2*entry], buf[2*entry+1]; cap_entries bounds the entries written.
+ * buf == NULL switches tracing off. */
+void hjo_set_trace(unsigned *buf, size_t cap_entries);
 
 #ifdef __cplusplus
 }
