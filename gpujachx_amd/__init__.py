@@ -25,6 +25,11 @@ NSWEEP = 30
 
 FAMILIES = ("mmstep", "BrentL", "colcyc", "cycloc", "rowcyc", "cycwor")
 
+# forms of the method (include/hypjac.h)
+FORM_PLAIN = 0
+FORM_LEARNED_SKIP = 1
+FORM_SPLIT_GRAM = 2
+
 _lib = None
 
 
@@ -61,12 +66,34 @@ def lib():
     L.hj_kernel_ms.restype = d
     L.hj_gram_nchunk.argtypes = [u, u]
     L.hj_gram_nchunk.restype = u
+    L.hj_set_max_sweeps.argtypes = [u]
+    L.hj_set_max_sweeps.restype = u
     L.hj_set_step_limit.argtypes = [ctypes.c_long]
     L.hj_set_step_limit.restype = None
     L.hj_dbg_scalar.argtypes = [i, i, pd, pd, pd, pd, pd, ctypes.POINTER(ctypes.c_int)]
     L.hj_dbg_scalar.restype = i
     L.hj_version.argtypes = []
     L.hj_version.restype = cp
+    pf = ctypes.POINTER(ctypes.c_float)
+    L.hj_set_form.argtypes = [u]
+    L.hj_set_form.restype = None
+    L.hj_set_skip_model.argtypes = [pf, u, ctypes.c_float, u]
+    L.hj_set_skip_model.restype = i
+    L.hj_mlp_nparam.restype = u
+    L.hj_mlp_nfeat.restype = u
+    L.hj_visit_record_size.restype = u
+    L.hj_set_visit_trace.argtypes = [vp, u]
+    L.hj_set_visit_trace.restype = None
+    L.hj_last_skips.argtypes = [pu, pull, pull]
+    L.hj_last_skips.restype = None
+    L.hj_skip_features.argtypes = [vp, u, ctypes.POINTER(ctypes.c_ushort), u, u, u, ctypes.c_float, pf]
+    L.hj_skip_features.restype = i
+    L.hj_mlp_grad.argtypes = [pf, pf, u, pf, ctypes.c_float, pf, pf]
+    L.hj_mlp_grad.restype = i
+    L.hj_mlp_predict.argtypes = [pf, u, pf, pf]
+    L.hj_mlp_predict.restype = i
+    L.hj_mlp_train.argtypes = [pf, pf, u, u, u, ctypes.c_float, ctypes.c_float, u, pf, pf]
+    L.hj_mlp_train.restype = i
     _lib = L
     return L
 
@@ -156,9 +183,9 @@ def hypJacL2(routine, G, nplus=None, ldhG=None, want_v=None):
 def last_sweeps():
     L = lib()
     n = ctypes.c_uint(0)
-    s = (ctypes.c_uint * NSWEEP)()
-    b = (ctypes.c_uint * NSWEEP)()
-    t = (ctypes.c_double * NSWEEP)()
+    s = (ctypes.c_uint * 256)()
+    b = (ctypes.c_uint * 256)()
+    t = (ctypes.c_double * 256)()
     L.hj_last_sweeps(ctypes.byref(n), s, b, t)
     k = n.value
     return list(s[:k]), list(b[:k]), list(t[:k])
@@ -174,8 +201,26 @@ def gram_nchunk(nrow, npairs):
     return lib().hj_gram_nchunk(nrow, npairs)
 
 
+def set_max_sweeps(n):
+    """Level-2 sweep cap (the reference's HYPJAC_NSWEEP = 30 by default)."""
+    return lib().hj_set_max_sweeps(int(n))
+
+
 def set_step_limit(steps):
     lib().hj_set_step_limit(int(steps))
+
+
+def set_form(form):
+    lib().hj_set_form(int(form))
+
+
+def last_skips():
+    """(per-sweep skipped visits, total skipped, total executed visits)."""
+    n, _, _ = len(last_sweeps()[0]), None, None
+    arr = (ctypes.c_uint * 256)()
+    sk, vi = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    lib().hj_last_skips(arr, ctypes.byref(sk), ctypes.byref(vi))
+    return list(arr[:n]), sk.value, vi.value
 
 
 def set_profile(on):

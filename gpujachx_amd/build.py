@@ -49,7 +49,7 @@ def _hdf5_prefix():
 def build_library():
     os.makedirs(LIB, exist_ok=True)
     objs = []
-    for src in ["hj_kernels.hip", "hj_host.cpp", "hj_strat.cpp"]:
+    for src in ["hj_kernels.hip", "hj_mlp.hip", "hj_host.cpp", "hj_strat.cpp"]:
         obj = os.path.join(LIB, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
@@ -83,7 +83,21 @@ def build_tools():
         return
     inc, lib = os.path.join(pre, "include"), os.path.join(pre, "lib")
     common = ["-O2", "-std=c++17", "-I" + inc, "-I" + os.path.join(ROOT, "include")]
-    link_h5 = ["-L" + lib, "-Wl,-rpath," + lib, "-lhdf5_hl", "-lhdf5"]
+    # The HDF5 prefix may also hold an older libstdc++ than the HIP runtime
+    # needs, so the tools do not search it: a private directory of links to
+    # the two HDF5 libraries is put on their run path instead.
+    h5dir = os.path.join(LIB, "hdf5")
+    os.makedirs(h5dir, exist_ok=True)
+    for stem in ("libhdf5", "libhdf5_hl"):
+        real = os.path.realpath(os.path.join(lib, stem + ".so"))
+        soname = subprocess.check_output(["objdump", "-p", real], text=True)
+        soname = [l.split()[-1] for l in soname.splitlines() if "SONAME" in l][0]
+        for name in (stem + ".so", soname):
+            dst = os.path.join(h5dir, name)
+            if os.path.lexists(dst):
+                os.remove(dst)
+            os.symlink(real, dst)
+    link_h5 = ["-L" + h5dir, "-Wl,-rpath,$ORIGIN/hdf5", "-lhdf5_hl", "-lhdf5"]
     srcs = {
         "x1": ["x1_main.cpp"],
         "h5gen": ["h5gen_main.cpp"],
@@ -96,6 +110,8 @@ def build_tools():
         extra = []
         if name == "x1":
             extra = ["-L" + LIB, "-Wl,-rpath,$ORIGIN", "-lhypjac"]
+        if name == "h5check":
+            extra = ["-fopenmp"]
         _run(["g++"] + common + ["-o", os.path.join(LIB, name)] + paths + extra + link_h5)
 
 

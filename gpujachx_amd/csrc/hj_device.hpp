@@ -34,92 +34,88 @@ __device__ __forceinline__ double rs_resid(double m, double y, double d)
   return T;
 }
 
+// Correctly rounded 1/sqrt(m) for m in [1,4): the hardware reciprocal square
+// root refined by two Newton steps y += y(1 - m y^2)/2 lands within an ulp
+// (the hardware seed has more than 20 correct bits, so the Newton error is
+// far below the final rounding); y then moves to a neighbour when the
+// midpoint between them lies on the other side of 1/sqrt(m), judged by the
+// sign of m*mid^2 - 1 with fma-exact products.  Straight-line code.
+__device__ __forceinline__ double rsqrt_core(double m)
+{
+  double y = __builtin_amdgcn_rsq(m);
+  double h = m * y;
+  y = __fma_rn(0.5 * y, __fma_rn(-h, y, 1.0), y);
+  h = m * y;
+  y = __fma_rn(0.5 * y, __fma_rn(-h, y, 1.0), y);
+  const double up = u2d(d2u(y) + 1u), dn = u2d(d2u(y) - 1u);
+  const double Tu = rs_resid(m, y, (up - y) * 0.5);
+  const double Td = rs_resid(m, y, -((y - dn) * 0.5));
+  return (Tu < 0.0) ? up : ((Td > 0.0) ? dn : y);
+}
+
 // Correctly rounded 1/sqrt(x).  Replaces my_drsqrt_rn of
 // share/device_code_common.hpp:89-133 (whose contract is the correctly
-// rounded result).  Own method: scale x to m in [1,4) by an even power of
-// two, take y = RN(1/RN(sqrt(m))) (within 2 ulps), then move to a neighbour
-// while the midpoint between y and it lies on the other side of 1/sqrt(m),
-// judged by the sign of m*mid^2 - 1 computed with fma-exact products.
+// rounded result).  Own method: x = m * 4^k with m in [1,4), rsqrt_core(m),
+// exact rescaling; IEEE special values as the reference returns them.
 __device__ __forceinline__ double dev_rsqrt(double x)
 {
   uint64_t b = d2u(x);
-  if (x != x) return x + x;
-  if (x == 0.0) return u2d((b & 0x8000000000000000ull) | 0x7ff0000000000000ull);
-  if (x < 0.0) return u2d(0xfff8000000000000ull);
-  if (b == 0x7ff0000000000000ull) return 0.0;
   int kadj = 0;
-  if (!(b >> 52)) {
-    x = x * 0x1p108;
+  if ((b >> 52) - 1u >= 0x7feu) {  // not a positive normal number
+    if (x != x) return x + x;
+    if (x == 0.0) return u2d((b & 0x8000000000000000ull) | 0x7ff0000000000000ull);
+    if (x < 0.0) return u2d(0xfff8000000000000ull);
+    if (b == 0x7ff0000000000000ull) return 0.0;
+    x = x * 0x1p108;  // positive subnormal
     b = d2u(x);
     kadj = 54;
   }
   const int e = (int)(b >> 52) - 1023;
   const int k = (e >= 0) ? (e / 2) : -((1 - e) / 2);
   const double m = u2d((b & 0x000fffffffffffffull) | ((uint64_t)(1023 + e - 2 * k) << 52));
-  double y = 1.0 / sqrt(m);
-  for (int it = 0; it < 4; ++it) {
-    const double up = u2d(d2u(y) + 1u);
-    if (rs_resid(m, y, (up - y) * 0.5) < 0.0) y = up;
-    else break;
-  }
-  for (int it = 0; it < 4; ++it) {
-    const double dn = u2d(d2u(y) - 1u);
-    if (rs_resid(m, y, -((y - dn) * 0.5)) > 0.0) y = dn;
-    else break;
-  }
-  const int s = kadj - k;
-  return y * u2d((uint64_t)(1023 + s) << 52);
+  const double y = rsqrt_core(m);
+  return y * u2d((uint64_t)(1023 + kadj - k) << 52);
 }
 
-// Trigonometric 2x2 rotation, share/device_code_common_rotate.hpp:189-213.
+// 2x2 rotation of share/device_code_common_rotate.hpp:189-237 (USE_RSQRT),
+// trigonometric (trig) or hyperbolic, evaluated without divergent branches:
+// every path's operations are issued and the taken branch's results are
+// selected, so the values equal the branch formulas bit for bit.
+//   trig: zeta = (Dq-Dp)/(2Apq); |zeta| < sqrt(eps/2): t = 1/(|zeta|+1),
+//         |zeta| >= sqrt(2/eps): t = 1/(2|zeta|), c = 1, else
+//         t = 1/(|zeta| + sqrt(zeta^2+1)); c = rsqrt(1+t^2); big = c < 1.
+//   hyp:  zeta = -(Dq+Dp)/(2Apq); |zeta| >= sqrt(2/eps): t = 1/(2|zeta|),
+//         c = 1; |zeta| > 1: t = 1/(|zeta| + sqrt(zeta^2-1)),
+//         c = rsqrt(1-t^2), big = c > 1; else c = 2/sqrt(3), t = 1/2.
+__device__ __forceinline__ int dev_rot(bool trig, double Apq, double Dp, double Dq, double &c,
+                                       double &t)
+{
+  const double h = trig ? (Dq - Dp) : (Dq + Dp);
+  const double qz = h / (2.0 * Apq);
+  const double ct2 = trig ? qz : -qz;
+  const double CT2 = fabs(ct2);
+  const bool large = CT2 >= SQRT_2_HEPS;
+  const bool small = trig && (CT2 < SQRT_HEPS);
+  const bool fix = !trig && !large && !(CT2 > 1.0);
+  const double sq = sqrt(__fma_rn(CT2, CT2, trig ? 1.0 : -1.0));
+  const double cot = large ? 2.0 * CT2 : (small ? CT2 + 1.0 : CT2 + sq);
+  const double tt = fix ? TANH1_FIX : copysign(1.0 / cot, ct2);
+  const double cr = dev_rsqrt(__fma_rn(trig ? tt : -tt, tt, 1.0));
+  c = large ? 1.0 : (fix ? COSHI_FIX : cr);
+  t = tt;
+  if (large) return 0;
+  if (small || fix) return 1;
+  return trig ? (cr < 1.0) : (cr > 1.0);
+}
+
 __device__ __forceinline__ int dev_rot_t(double Apq, double Dp, double Dq, double &c, double &t)
 {
-  const double h = Dq - Dp;
-  const double ct2 = h / (2.0 * Apq);
-  const double CT2 = fabs(ct2);
-  if (CT2 < SQRT_HEPS) {
-    const double cot = CT2 + 1.0;
-    t = copysign(1.0 / cot, ct2);
-    c = dev_rsqrt(__fma_rn(t, t, 1.0));
-    return 1;
-  }
-  else if (CT2 >= SQRT_2_HEPS) {
-    const double cot = 2.0 * CT2;
-    c = 1.0;
-    t = copysign(1.0 / cot, ct2);
-    return 0;
-  }
-  else {
-    const double cot = CT2 + sqrt(__fma_rn(CT2, CT2, 1.0));
-    t = copysign(1.0 / cot, ct2);
-    c = dev_rsqrt(__fma_rn(t, t, 1.0));
-    return (c < 1.0);
-  }
+  return dev_rot(true, Apq, Dp, Dq, c, t);
 }
 
-// Hyperbolic 2x2 rotation, share/device_code_common_rotate.hpp:214-237.
 __device__ __forceinline__ int dev_rot_h(double Apq, double Dp, double Dq, double &c, double &t)
 {
-  const double h = Dq + Dp;
-  const double ct2 = -(h / (2.0 * Apq));
-  const double CT2 = fabs(ct2);
-  if (CT2 >= SQRT_2_HEPS) {
-    const double cot = 2.0 * CT2;
-    c = 1.0;
-    t = copysign(1.0 / cot, ct2);
-    return 0;
-  }
-  else if (CT2 > 1.0) {
-    const double cot = CT2 + sqrt(__fma_rn(CT2, CT2, -1.0));
-    t = copysign(1.0 / cot, ct2);
-    c = dev_rsqrt(__fma_rn(-t, t, 1.0));
-    return (c > 1.0);
-  }
-  else {
-    c = COSHI_FIX;
-    t = TANH1_FIX;
-    return 1;
-  }
+  return dev_rot(false, Apq, Dp, Dq, c, t);
 }
 
 // xor-swizzle of a double inside each 32-lane half of the wave
@@ -143,6 +139,39 @@ __device__ __forceinline__ double sum32(double v)
   v = v + swz_xor<2>(v);
   v = v + swz_xor<1>(v);
   return v;
+}
+
+// Lane exchanges inside groups of 8 lanes by DPP (no LDS traffic):
+// xor 1 and xor 2 are quad permutations, xor 4 is the half-row mirror
+// (lane i <- 7-i) followed by the quad reversal (i <- 3-i).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_xor4(double v) { return dpp_mov<0x1B>(dpp_mov<0x141>(v)); }
+__device__ __forceinline__ double lane_xor2(double v) { return dpp_mov<0x4E>(v); }
+__device__ __forceinline__ double lane_xor1(double v) { return dpp_mov<0xB1>(v); }
+
+// Sum of the 32 rows of one pivot pair held by 8 lanes, lane r holding rows
+// r, r+8, r+16, r+24 as v0..v3: the xor butterfly 16,8,4,2,1 of the 32-lane
+// reduction (levels 16 and 8 inside the lane, 4,2,1 across lanes).
+__device__ __forceinline__ double sum8x4(double v0, double v1, double v2, double v3)
+{
+  double s = (v0 + v2) + (v1 + v3);
+  s = s + lane_xor4(s);
+  s = s + lane_xor2(s);
+  s = s + lane_xor1(s);
+  return s;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
 }
 
 }  // namespace hj

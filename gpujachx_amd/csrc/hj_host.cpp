@@ -2,6 +2,8 @@
 // thin level-1 layer and device configuration, on the HIP runtime.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -10,22 +12,36 @@
 
 #include "../../include/hypjac.h"
 #include "hj_kernels.hpp"
+#include "hj_mlp.hpp"
 #include "hj_strat.hpp"
 #include "hj_util.hpp"
 
 namespace hj {
 
+#define HJ_MAX_SWEEP_CAP 256u
 struct Stats {
   unsigned nswp = 0u;
-  unsigned rot_s[HYPJAC_NSWEEP] = {0u};
-  unsigned rot_b[HYPJAC_NSWEEP] = {0u};
-  double sec[HYPJAC_NSWEEP] = {0.0};
+  unsigned rot_s[HJ_MAX_SWEEP_CAP] = {0u};
+  unsigned rot_b[HJ_MAX_SWEEP_CAP] = {0u};
+  double sec[HJ_MAX_SWEEP_CAP] = {0.0};
+  unsigned skip[HJ_MAX_SWEEP_CAP] = {0u};
   double kms[3] = {0.0, 0.0, 0.0};
   unsigned long long kcnt[3] = {0ull, 0ull, 0ull};
+  unsigned long long skipped = 0ull, visits = 0ull;
 };
 static Stats g_stats;
 static int g_profile = 0;
 static long g_step_limit = 0;
+static unsigned g_max_sweeps = HYPJAC_NSWEEP;  // level-2 sweep cap
+// forms and the pair-skip model
+static unsigned g_form = 0u;
+static bool g_model_ready = false;
+static uint16_t g_model_bf16[MLP_NPARAM];
+static float g_skip_threshold = 0.05f;
+static unsigned g_skip_max = 2u;
+// per-sweep visit records copied to the host (trace collection)
+static void *g_trace_buf = nullptr;
+static unsigned g_trace_sweeps = 0u;
 
 static bool quiet()
 {
@@ -61,8 +77,13 @@ struct Work {
   uint16_t *d_s1 = nullptr;
   uint8_t *d_s0 = nullptr;
   double *d_part = nullptr, *d_W = nullptr;
-  unsigned *d_flags = nullptr, *d_pstat = nullptr;
-  unsigned long long *d_cvg = nullptr, *h_cvg = nullptr;
+  unsigned *d_flags = nullptr;
+  RunCtl *d_ctl = nullptr, *h_ctl = nullptr;
+  VisitRec *d_rec = nullptr;   // [steps][np] last executed visit of each pair
+  uint8_t *d_skip = nullptr;   // [steps][np] skip mask of the running sweep
+  uint16_t *d_model = nullptr; // bf16 MLP parameters
+
+  size_t keys() const { return (size_t)np * steps; }
 
   void alloc(unsigned nrow, unsigned ncol, bool withV)
   {
@@ -78,9 +99,11 @@ struct Work {
     HIP_CALL(hipMalloc(&d_part, sizeof(double) * 768u * nchunk * np));
     HIP_CALL(hipMalloc(&d_W, sizeof(double) * 1024u * np));
     HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np));
-    HIP_CALL(hipMalloc(&d_pstat, sizeof(unsigned) * 2u * np));
-    HIP_CALL(hipMalloc(&d_cvg, sizeof(unsigned long long)));
-    HIP_CALL(hipHostMalloc(&h_cvg, sizeof(unsigned long long), hipHostMallocDefault));
+    HIP_CALL(hipMalloc(&d_ctl, sizeof(RunCtl)));
+    HIP_CALL(hipHostMalloc(&h_ctl, sizeof(RunCtl), hipHostMallocDefault));
+    HIP_CALL(hipMalloc(&d_rec, sizeof(VisitRec) * keys()));
+    HIP_CALL(hipMalloc(&d_skip, keys()));
+    HIP_CALL(hipMalloc(&d_model, sizeof(uint16_t) * MLP_NPARAM));
     std::vector<uint16_t> s1((size_t)2u * np * steps);
     for (unsigned s = 0u; s < steps; ++s)
       for (unsigned p = 0u; p < np; ++p) {
@@ -95,6 +118,14 @@ struct Work {
         s0[(s * 16u + p) * 2u + 1u] = strat0[s][p][1];
       }
     HIP_CALL(hipMemcpy(d_s0, s0.data(), STRAT0_TAB_BYTES, hipMemcpyHostToDevice));
+    // records start as "never visited", no pair skipped, counters zero
+    std::vector<VisitRec> r(keys());
+    memset(r.data(), 0, sizeof(VisitRec) * r.size());
+    HIP_CALL(hipMemcpy(d_rec, r.data(), sizeof(VisitRec) * r.size(), hipMemcpyHostToDevice));
+    std::vector<uint8_t> z(keys(), 0u);
+    HIP_CALL(hipMemcpy(d_skip, z.data(), keys(), hipMemcpyHostToDevice));
+    memset(h_ctl, 0, sizeof(RunCtl));
+    HIP_CALL(hipMemcpy(d_ctl, h_ctl, sizeof(RunCtl), hipMemcpyHostToDevice));
   }
   void release()
   {
@@ -103,9 +134,11 @@ struct Work {
     HIP_CALL(hipFree(d_part));
     HIP_CALL(hipFree(d_W));
     HIP_CALL(hipFree(d_flags));
-    HIP_CALL(hipFree(d_pstat));
-    HIP_CALL(hipFree(d_cvg));
-    HIP_CALL(hipHostFree(h_cvg));
+    HIP_CALL(hipFree(d_ctl));
+    HIP_CALL(hipHostFree(h_ctl));
+    HIP_CALL(hipFree(d_rec));
+    HIP_CALL(hipFree(d_skip));
+    HIP_CALL(hipFree(d_model));
   }
 };
 
@@ -120,20 +153,29 @@ static void check_tables(unsigned ncol)
 // One level-1 step: Gram partials, pair kernels, block update.
 static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, unsigned nswp,
                        int definite, unsigned nplus, double *dG, unsigned lddG, unsigned nrow,
-                       double *dV, unsigned lddV, unsigned ncol, hipEvent_t *ev)
+                       double *dV, unsigned lddV, unsigned ncol, hipEvent_t *ev, bool skipping)
 {
-  const uint16_t *pairs = w.d_s1 + (size_t)step * w.np * 2u;
+  const size_t off = (size_t)step * w.np;
+  const uint16_t *pairs = w.d_s1 + off * 2u;
+  const uint8_t *skip = skipping ? w.d_skip + off : nullptr;
   if (ev) HIP_CALL(hipEventRecord(ev[0], s));
-  launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part);
+  launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
   if (ev) HIP_CALL(hipEventRecord(ev[1], s));
-  launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_cvg, w.d_pstat,
-              w.d_s0, w.s0steps, nswp, definite, nplus);
+  launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
+              skip, w.d_s0, w.s0steps, nswp, definite, nplus);
   if (ev) HIP_CALL(hipEventRecord(ev[2], s));
   launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
   if (ev) HIP_CALL(hipEventRecord(ev[3], s));
 }
 
 // Sweeps + initD on device data (the body of 1/HypJacL2.cu:104-178).
+//
+// With the learned pair-skip form, every sweep after the first starts with
+// the predictor, which marks the visits whose model probability of a big
+// rotation is below the threshold; those visits cost nothing.  A sweep with
+// skipped visits never ends the run: when it finds no big rotation, the next
+// sweep runs unskipped (the certification sweep), and only an unskipped sweep
+// without a big rotation ends it -- the reference's test (1/HypJacL2.cu:167).
 static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
                         unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
                         unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t caller)
@@ -161,6 +203,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     launch_initV(s, dV, 0u, ncol, lddV);
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
   double *const V = full_svd ? dV : nullptr;
+  const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready;
+  if (learned)
+    HIP_CALL(hipMemcpy(w.d_model, g_model_bf16, sizeof(uint16_t) * MLP_NPARAM, hipMemcpyHostToDevice));
+  const float log2n1 = (float)std::log2((double)(ncol / 16u));
+  const float max_rot = (float)w.keys() * 496.0f;
 
   g_stats = Stats();
   *glb_s = 0ull;
@@ -172,15 +219,22 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     evs.resize((size_t)4u * w.steps);
     for (auto &e : evs) HIP_CALL(hipEventCreate(&e));
   }
+  auto sweep_head = [&](hipStream_t st) {
+    launch_sweep_begin(st, w.d_ctl);
+    if (learned)
+      launch_skip_infer(st, w.d_rec, (unsigned)w.keys(), w.d_s1, nplus, log2n1, max_rot, w.d_ctl,
+                        w.d_model, w.d_skip);
+  };
   if (!eager) {
-    // one sweep as a graph: zero the counter (a kernel, no memset node),
-    // all steps, copy the counter to pinned host memory
+    // one sweep as a graph: start-of-sweep kernel (counters, no memset
+    // node), the predictor, all steps, the control block to pinned memory
     hipGraph_t graph;
     HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_zero_u64(s, w.d_cvg);
+    sweep_head(s);
     for (unsigned st = 0u; st < w.steps; ++st)
-      issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr);
-    HIP_CALL(hipMemcpyAsync(w.h_cvg, w.d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
+                 learned);
+    HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamEndCapture(s, &graph));
     HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     HIP_CALL(hipGraphDestroy(graph));
@@ -190,10 +244,22 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   stopwatch_reset(swp_tim);
   unsigned blk_swp = 0u;
   long steps_done = 0;
-  bool stop = false;
-  while (blk_swp < HYPJAC_NSWEEP) {
+  bool stop = false, certify = false;
+  g_stats.skipped = 0ull;
+  while (blk_swp < g_max_sweeps) {
+    if (learned) {
+      // skipping on unless this is the first or a certification sweep
+      RunCtl c;
+      c.skip_on = (blk_swp > 0u && !certify) ? 1u : 0u;
+      c.threshold = g_skip_threshold;
+      c.max_skips = g_skip_max;
+      HIP_CALL(hipMemcpyAsync(reinterpret_cast<char *>(w.d_ctl) + offsetof(RunCtl, skip_on), &c.skip_on,
+                              offsetof(RunCtl, skipped) - offsetof(RunCtl, skip_on),
+                              hipMemcpyHostToDevice, s));
+      HIP_CALL(hipStreamSynchronize(s));
+    }
     if (eager) {
-      launch_zero_u64(s, w.d_cvg);
+      sweep_head(s);
       for (unsigned st = 0u; st < w.steps; ++st) {
         if (g_step_limit > 0 && steps_done >= g_step_limit) {
           stop = true;
@@ -201,9 +267,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
         }
         ++steps_done;
         issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
-                   g_profile ? &evs[(size_t)4u * st] : nullptr);
+                   g_profile ? &evs[(size_t)4u * st] : nullptr, learned);
       }
-      HIP_CALL(hipMemcpyAsync(w.h_cvg, w.d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     }
     else
       HIP_CALL(hipGraphLaunch(gexec, s));
@@ -217,23 +283,41 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
           g_stats.kcnt[k] += 1ull;
         }
     ++blk_swp;
-    const unsigned long long cvg = *w.h_cvg;
+    const unsigned long long cvg = w.h_ctl->cvg;
+    const unsigned skipped = w.h_ctl->skipped;
     const unsigned cvg_s = static_cast<unsigned>(cvg);
     const unsigned cvg_b = static_cast<unsigned>(cvg >> 32u);
     *glb_s += cvg_s;
     *glb_b += cvg_b;
     const double tim_s = stopwatch_lap(swp_tim) * TS2S;
-    g_stats.rot_s[blk_swp - 1u] = cvg_s;
-    g_stats.rot_b[blk_swp - 1u] = cvg_b;
-    g_stats.sec[blk_swp - 1u] = tim_s;
+    if (blk_swp <= HJ_MAX_SWEEP_CAP) {
+      g_stats.rot_s[blk_swp - 1u] = cvg_s;
+      g_stats.rot_b[blk_swp - 1u] = cvg_b;
+      g_stats.sec[blk_swp - 1u] = tim_s;
+      g_stats.skip[blk_swp - 1u] = skipped;
+    }
     g_stats.nswp = blk_swp;
+    g_stats.skipped += skipped;
+    g_stats.visits += w.keys() - skipped;
+    if (g_trace_buf && blk_swp <= g_trace_sweeps) {
+      HIP_CALL(hipMemcpy(static_cast<char *>(g_trace_buf) + (size_t)(blk_swp - 1u) * w.keys() * sizeof(VisitRec),
+                         w.d_rec, w.keys() * sizeof(VisitRec), hipMemcpyDeviceToHost));
+    }
     if (!quiet()) {
       (void)fprintf(stdout, "BLK_SWP(%2u), ROT_S(%10u), ROT_B(%10u), TIME(%#12.6f s)\n", blk_swp,
                     cvg_s, cvg_b, tim_s);
       (void)fflush(stdout);
     }
-    if (!cvg_b || stop)
+    if (stop)
       break;
+    if (!cvg_b) {
+      if (skipped) {
+        certify = true;  // the next sweep runs every visit
+        continue;
+      }
+      break;
+    }
+    certify = false;
   }
   if (gexec) HIP_CALL(hipGraphExecDestroy(gexec));
   for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
@@ -278,14 +362,24 @@ struct Thin {
 };
 static Thin g_thin;
 
+// The rotation counter of the thin layer is the caller's 8-byte device
+// word: the step's count (added by the pair kernels to the workspace's
+// control block) is added to it after the step.
+__global__ void k_thin_accum(hj::RunCtl *ctl, unsigned long long *cvg)
+{
+  if (threadIdx.x == 0u) {
+    *cvg += ctl->cvg;
+    ctl->cvg = 0ull;
+  }
+}
+
 static void thin_step(unsigned step, bool sort, bool withV, int definite, unsigned nplus)
 {
   if (!g_thin.ready) HJ_DIE("initSymbols not called");
   if (step >= g_thin.w.steps) HJ_DIE("step out of range");
-  Work w = g_thin.w;
-  w.d_cvg = g_thin.cvg;
-  issue_step(nullptr, w, step, sort, g_thin.nSwp, definite, nplus, g_thin.G, g_thin.ldG,
-             g_thin.nRow, withV ? g_thin.V : nullptr, g_thin.ldV, g_thin.nCol, nullptr);
+  issue_step(nullptr, g_thin.w, step, sort, g_thin.nSwp, definite, nplus, g_thin.G, g_thin.ldG,
+             g_thin.nRow, withV ? g_thin.V : nullptr, g_thin.ldV, g_thin.nCol, nullptr, false);
+  hipLaunchKernelGGL(k_thin_accum, dim3(1), dim3(64), 0, nullptr, g_thin.w.d_ctl, g_thin.cvg);
   HIP_CALL(hipGetLastError());
 }
 
@@ -381,6 +475,39 @@ extern "C" void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
 
 extern "C" void hj_set_profile(int on) { g_profile = on; }
 
+extern "C" void hj_set_form(unsigned form) { g_form = form; }
+
+extern "C" int hj_set_skip_model(const float *params, unsigned nparam, float threshold,
+                                 unsigned max_skips)
+{
+  if (!params || nparam != MLP_NPARAM) return -1;
+  for (unsigned i = 0u; i < MLP_NPARAM; ++i) {
+    // bfloat16, round to nearest even
+    uint32_t u;
+    memcpy(&u, &params[i], 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    g_model_bf16[i] = (uint16_t)(u >> 16);
+  }
+  g_skip_threshold = threshold;
+  g_skip_max = max_skips;
+  g_model_ready = true;
+  return 0;
+}
+
+extern "C" void hj_set_visit_trace(void *host_buf, unsigned max_sweeps)
+{
+  g_trace_buf = host_buf;
+  g_trace_sweeps = host_buf ? max_sweeps : 0u;
+}
+
+extern "C" void hj_last_skips(unsigned *skip, unsigned long long *skipped, unsigned long long *visits)
+{
+  for (unsigned i = 0u; i < g_stats.nswp && i < HJ_MAX_SWEEP_CAP; ++i)
+    if (skip) skip[i] = g_stats.skip[i];
+  if (skipped) *skipped = g_stats.skipped;
+  if (visits) *visits = g_stats.visits;
+}
+
 extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
 {
   if (kind < 0 || kind > 2) return -1.0;
@@ -391,6 +518,14 @@ extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
 extern "C" unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs) { return gram_nchunk(nrow, npairs); }
 
 extern "C" void hj_set_step_limit(long steps) { g_step_limit = steps; }
+
+extern "C" unsigned hj_set_max_sweeps(unsigned n)
+{
+  const unsigned old = g_max_sweeps;
+  if (n >= 1u && n <= HJ_MAX_SWEEP_CAP)
+    g_max_sweeps = n;
+  return old;
+}
 
 extern "C" int hj_dbg_scalar(int kind, int n, const double *a, const double *b, const double *c,
                              double *out_c, double *out_t, int *out_big)

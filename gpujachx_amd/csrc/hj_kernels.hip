@@ -5,8 +5,8 @@
 //   k_gram   : partial 32x32 Gram of each block pair over a row chunk, on
 //              FP64 MFMA 16x16x4 (one wave per (pair, chunk));
 //   k_pair   : per pair, sum of the partials, 32x32 Cholesky, the inner
-//              Jacobi sweep(s) on R in LDS accumulating W (one 512-thread
-//              workgroup per pair, one pivot pair per 32-lane half wave);
+//              Jacobi sweep(s) on R in LDS accumulating W (one 128-thread
+//              workgroup per pair, 8 lanes per pivot pair);
 //   k_update : [G_i G_j] <- [G_i G_j] W and [V_i V_j] <- [V_i V_j] W on FP64
 //              MFMA (one wave per (pair, row chunk)).
 // The reference fuses the three into one CUDA block per pair
@@ -25,71 +25,122 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 }
 
 // ---------------------------------------------------------------------------
-// Gram partials.  Wave (pair pi, chunk c) covers rows [c*CH, (c+1)*CH).
-// Lane l owns column l%16 of both blocks; per 64-row group it loads rows
-// 16*(l/16)+t, t=0..15 (128 contiguous bytes per column per lane), and MFMA
-// step t takes, for k = l/16, row 16k+t of the group.  Tiles: 00 = block i
-// with itself, 10 = block j with block i, 11 = block j with itself.
-__global__ __launch_bounds__(64) void k_gram(const double *__restrict__ G, unsigned ldG,
-                                             const uint16_t *__restrict__ pairs, unsigned nchunk,
-                                             unsigned CH, double *__restrict__ part)
+// Gram partials.  Workgroup (pair pi, chunk c) covers rows [c*CH, (c+1)*CH)
+// with nw waves; wave w takes the sub-chunk of CH/nw rows starting at
+// c*CH + w*CH/nw, and the nw wave results are added in wave order through
+// LDS.  Within a wave, lane l owns column l%16 of both blocks (k = l/16);
+// for a 64-row group, load j (j = 0..7) brings rows 8j+2k and 8j+2k+1, so
+// the four lanes of a column read 64 contiguous bytes per instruction, and
+// MFMA steps (j, e) take row 8j+2k+e in k-slot k.  The next group's loads are
+// in flight while this group's MFMAs run.  Tiles: 00 = block i with itself,
+// 10 = block j with block i, 11 = block j with itself.
+__global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsigned ldG,
+                                              const uint16_t *__restrict__ pairs, unsigned nchunk,
+                                              unsigned CH, double *__restrict__ part,
+                                              const uint8_t *__restrict__ skip)
 {
-  const unsigned l = threadIdx.x;
+  __shared__ d4 red[3][GRAM_WAVES - 1][64];
+  const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const unsigned pi = blockIdx.x / nchunk, c = blockIdx.x - pi * nchunk;
+  if (skip && skip[pi])
+    return;
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
-  const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + (size_t)c * CH + 16u * (l >> 4);
-  const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + (size_t)c * CH + 16u * (l >> 4);
+  const unsigned SUB = CH / nw;
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 2u * (l >> 4);
+  const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
+  const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
-  for (unsigned g = 0u; g < CH; g += 64u) {
-    double x0[16], x1[16];
-    const double2 *const p0 = reinterpret_cast<const double2 *>(g0 + g);
-    const double2 *const p1 = reinterpret_cast<const double2 *>(g1 + g);
+  double2 n0[8], n1[8];
 #pragma unroll
-    for (unsigned t = 0u; t < 8u; ++t) {
-      const double2 v0 = p0[t], v1 = p1[t];
-      x0[2u * t] = v0.x;
-      x0[2u * t + 1u] = v0.y;
-      x1[2u * t] = v1.x;
-      x1[2u * t + 1u] = v1.y;
+  for (unsigned j = 0u; j < 8u; ++j) {
+    n0[j] = *reinterpret_cast<const double2 *>(g0 + 8u * j);
+    n1[j] = *reinterpret_cast<const double2 *>(g1 + 8u * j);
+  }
+  for (unsigned g = 0u; g < SUB; g += 64u) {
+    double2 x0[8], x1[8];
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      x0[j] = n0[j];
+      x1[j] = n1[j];
+    }
+    if (g + 64u < SUB) {
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j) {
+        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 64u + 8u * j);
+        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 64u + 8u * j);
+      }
     }
 #pragma unroll
-    for (unsigned t = 0u; t < 16u; ++t) {
-      a00 = mfma(x0[t], x0[t], a00);
-      a10 = mfma(x1[t], x0[t], a10);
-      a11 = mfma(x1[t], x1[t], a11);
+    for (unsigned j = 0u; j < 8u; ++j) {
+      a00 = mfma(x0[j].x, x0[j].x, a00);
+      a10 = mfma(x1[j].x, x0[j].x, a10);
+      a11 = mfma(x1[j].x, x1[j].x, a11);
+      a00 = mfma(x0[j].y, x0[j].y, a00);
+      a10 = mfma(x1[j].y, x0[j].y, a10);
+      a11 = mfma(x1[j].y, x1[j].y, a11);
     }
   }
-  double *const o = part + (size_t)blockIdx.x * 768u + l * 4u;
+  if (w) {
+    red[0][w - 1u][l] = a00;
+    red[1][w - 1u][l] = a10;
+    red[2][w - 1u][l] = a11;
+  }
+  __syncthreads();
+  if (w == 0u) {
+    for (unsigned v = 1u; v < nw; ++v) {
+      a00 = a00 + red[0][v - 1u][l];
+      a10 = a10 + red[1][v - 1u][l];
+      a11 = a11 + red[2][v - 1u][l];
+    }
+    double *const o = part + (size_t)blockIdx.x * 768u + l * 4u;
 #pragma unroll
-  for (unsigned r = 0u; r < 4u; ++r) {
-    o[r] = a00[r];
-    o[256u + r] = a10[r];
-    o[512u + r] = a11[r];
+    for (unsigned r = 0u; r < 4u; ++r) {
+      o[r] = a00[r];
+      o[256u + r] = a10[r];
+      o[512u + r] = a11[r];
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Block pair: Gram -> Cholesky -> inner Jacobi -> W.
-// Thread t: x = t%32 (row of R and W), y = t/32 (pivot-pair slot 0..15).
+// Block pair: Gram -> Cholesky -> inner Jacobi -> W, two waves per pair.
+// R and W live in LDS, column-major with a padded leading dimension.
+// Cholesky: wave 0, lane x (x < 32) keeps row x of the Gram in registers;
+// the pivot column is broadcast with readlane, no barriers.
+// Inner Jacobi: pivot-pair slot y = t/8 (0..15); lane r = t%8 of the slot
+// keeps rows r, r+8, r+16, r+24 of the two columns of R and of W.
 
+#define LDR 33u
+#define RA(A, r, c) (A)[(c) * LDR + (r)]
 #define F32(A, r, c) (A)[(c) * 32u + (r)]
 
 template <bool SORT>
-__global__ __launch_bounds__(512) void k_pair(const uint16_t *__restrict__ pairs,
+__global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs,
                                               const double *__restrict__ part, unsigned nchunk,
                                               double *__restrict__ Wout, unsigned *__restrict__ flags,
-                                              unsigned long long *__restrict__ cvg,
-                                              unsigned *__restrict__ pstat,
+                                              RunCtl *__restrict__ ctl, VisitRec *__restrict__ rec,
+                                              const uint8_t *__restrict__ skip,
                                               const uint8_t *__restrict__ strat0, unsigned s0steps,
                                               unsigned nswp, int definite, unsigned nplus)
 {
-  __shared__ double R[1024];
-  __shared__ double W[1024];
+  __shared__ double R[32u * LDR];
+  __shared__ double W[32u * LDR];
   __shared__ uint8_t S0[STRAT0_TAB_BYTES];
   __shared__ unsigned cnt_s[16], cnt_b[16];
+  __shared__ float cmax[16];
+  __shared__ float bnorm[2];
 
-  const unsigned tid = threadIdx.x, x = tid & 31u, y = tid >> 5;
+  const unsigned tid = threadIdx.x;
   const unsigned pi = blockIdx.x;
+  if (skip && skip[pi]) {
+    // skipped visit: no update, the record keeps the last executed visit
+    if (tid == 0u) {
+      flags[pi] = 0u;
+      rec[pi].skips += 1u;
+      atomicAdd(&ctl->skipped, 1u);
+    }
+    return;
+  }
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
 
   // local J structure, share/device_code_nosort_accumV.hpp:53-76
@@ -106,68 +157,96 @@ __global__ __launch_bounds__(512) void k_pair(const uint16_t *__restrict__ pairs
     hyp = (def == 0);
   }
 
-  for (unsigned e = tid; e < s0steps * 32u; e += 512u)
+  for (unsigned e = tid; e < s0steps * 32u; e += 128u)
     S0[e] = strat0[e];
 
-  // sum of the chunk partials in chunk order
+  // sum of the chunk partials in chunk order (all loads issued first)
   const double *const pp = part + (size_t)pi * nchunk * 768u;
-  for (unsigned e = tid; e < 768u; e += 512u) {
-    double s = pp[e];
-    for (unsigned c = 1u; c < nchunk; ++c)
-      s = s + pp[(size_t)c * 768u + e];
+  for (unsigned e = tid; e < 768u; e += 128u) {
+    double v[MAX_GRAM_CHUNKS];
+#pragma unroll
+    for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
+      if (c < nchunk) v[c] = pp[(size_t)c * 768u + e];
+    double s = v[0];
+#pragma unroll
+    for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
+      if (c < nchunk) s = s + v[c];
     const unsigned tile = e >> 8, l = (e >> 2) & 63u, r = e & 3u;
     const unsigned i = 4u * r + (l >> 4), j = l & 15u;
     const unsigned row = (tile == 0u) ? i : (16u + i);
     const unsigned col = (tile == 2u) ? (16u + j) : j;
-    F32(R, row, col) = s;
+    RA(R, row, col) = s;
   }
-  F32(W, x, y) = (x == y) ? 1.0 : 0.0;
-  F32(W, x, y + 16u) = (x == y + 16u) ? 1.0 : 0.0;
+  for (unsigned e = tid; e < 1024u; e += 128u)
+    RA(W, e & 31u, e >> 5) = ((e & 31u) == (e >> 5)) ? 1.0 : 0.0;
   __syncthreads();
+  if (tid < 2u) {
+    // squared Frobenius norms of the two blocks, for the visit record
+    double sn = 0.0;
+    for (unsigned k = 0u; k < 16u; ++k)
+      sn = sn + RA(R, 16u * tid + k, 16u * tid + k);
+    bnorm[tid] = (float)log2(sn);
+  }
 
   // Cholesky, share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT):
   // column k scaled by rsqrt(A(k,k)), diagonal sqrt(A(k,k)), then
-  // A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k.
-  for (unsigned k = 0u; k < 32u; ++k) {
-    if (tid < 32u && x >= k) {
-      const double Akk = F32(R, k, k);
-      const double Axk = F32(R, x, k);
-      F32(R, x, k) = (x > k) ? (Axk * dev_rsqrt(Akk)) : sqrt(Akk);
+  // A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k; R = L^T.
+  if (tid < 64u) {
+    const unsigned x = tid & 31u;
+    double a[32];
+#pragma unroll
+    for (unsigned j = 0u; j < 32u; ++j)
+      a[j] = (j <= x) ? RA(R, x, j) : 0.0;
+#pragma unroll
+    for (unsigned k = 0u; k < 32u; ++k) {
+      const double akk = readlane_d(a[k], k);
+      const double rk = dev_rsqrt(akk);
+      const double dk = sqrt(akk);
+      if (x > k) a[k] = a[k] * rk;
+      else if (x == k) a[k] = dk;
+#pragma unroll
+      for (unsigned j = k + 1u; j < 32u; ++j) {
+        const double ljk = readlane_d(a[k], j);
+        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
+      }
     }
-    __syncthreads();
-    const unsigned j0 = k + 1u + y, j1 = j0 + 16u;
-    if (j0 < 32u && x >= j0)
-      F32(R, x, j0) = __fma_rn(-F32(R, x, k), F32(R, j0, k), F32(R, x, j0));
-    if (j1 < 32u && x >= j1)
-      F32(R, x, j1) = __fma_rn(-F32(R, x, k), F32(R, j1, k), F32(R, x, j1));
-    __syncthreads();
+    if (tid < 32u) {
+#pragma unroll
+      for (unsigned y = 0u; y < 32u; ++y)
+        RA(R, y, x) = (y <= x) ? a[y] : 0.0;
+    }
   }
-  {
-    // R = L^T with zeros below the diagonal
-    const double a0 = (x >= y) ? F32(R, x, y) : 0.0;
-    const double a1 = (x >= y + 16u) ? F32(R, x, y + 16u) : 0.0;
-    __syncthreads();
-    F32(R, y, x) = a0;
-    F32(R, y + 16u, x) = a1;
-    __syncthreads();
-  }
+  __syncthreads();
 
   // Inner Jacobi: share/device_code_nosort.hpp:4-222 (SORT false),
   // share/device_code_cdsort_KISS.hpp:4-260 and share/device_code_cdsort.hpp:25-167
   // (SORT true).  One barrier per step: the pivot pairs of a step touch
   // disjoint columns; rotation counts are kept per slot and summed per sweep.
+  const unsigned y = tid >> 3, r = tid & 7u;
   unsigned blk_s = 0u, blk_b = 0u;
+  double my_cos = 0.0;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned my_s = 0u, my_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
       const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
-      const double Gp = F32(R, x, p), Gq = F32(R, x, q);
-      const double Vp = F32(W, x, p), Vq = F32(W, x, q);
-      double Dp = sum32(__fma_rn(Gp, Gp, 0.0));
-      double Dq = sum32(__fma_rn(Gq, Gq, 0.0));
-      const double Apq = sum32(__fma_rn(Gp, Gq, 0.0));
+      double Gp[4], Gq[4], Vp[4], Vq[4];
+#pragma unroll
+      for (unsigned i = 0u; i < 4u; ++i) {
+        Gp[i] = RA(R, r + 8u * i, p);
+        Gq[i] = RA(R, r + 8u * i, q);
+        Vp[i] = RA(W, r + 8u * i, p);
+        Vq[i] = RA(W, r + 8u * i, q);
+      }
+      double Dp = sum8x4(__fma_rn(Gp[0], Gp[0], 0.0), __fma_rn(Gp[1], Gp[1], 0.0),
+                         __fma_rn(Gp[2], Gp[2], 0.0), __fma_rn(Gp[3], Gp[3], 0.0));
+      double Dq = sum8x4(__fma_rn(Gq[0], Gq[0], 0.0), __fma_rn(Gq[1], Gq[1], 0.0),
+                         __fma_rn(Gq[2], Gq[2], 0.0), __fma_rn(Gq[3], Gq[3], 0.0));
+      const double Apq = sum8x4(__fma_rn(Gp[0], Gq[0], 0.0), __fma_rn(Gp[1], Gq[1], 0.0),
+                                __fma_rn(Gp[2], Gq[2], 0.0), __fma_rn(Gp[3], Gq[3], 0.0));
       const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
       const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
+      if (swp == 0u)
+        my_cos = fmax(my_cos, (Dpq_ > 0.0) ? (Apq_ / Dpq_) : 0.0);
       int pdef = def;
       if (hyp) {
         if ((p < npos) && (q < npos)) pdef = +1;
@@ -175,65 +254,46 @@ __global__ __launch_bounds__(512) void k_pair(const uint16_t *__restrict__ pairs
         else pdef = 0;
       }
       int transf_b = 0;
-      if (!SORT) {
-        if (transf_s) {
-          double c, t, t_;
-          if (!hyp || pdef) { transf_b = dev_rot_t(Apq, Dp, Dq, c, t); t_ = -t; }
-          else { transf_b = dev_rot_h(Apq, Dp, Dq, c, t); t_ = t; }
-          if (transf_b) {
-            F32(R, x, p) = c * __fma_rn(t_, Gq, Gp);
-            F32(R, x, q) = c * __fma_rn(t, Gp, Gq);
-            F32(W, x, p) = c * __fma_rn(t_, Vq, Vp);
-            F32(W, x, q) = c * __fma_rn(t, Vp, Vq);
-          }
-          else {
-            F32(R, x, p) = __fma_rn(t_, Gq, Gp);
-            F32(R, x, q) = __fma_rn(t, Gp, Gq);
-            F32(W, x, p) = __fma_rn(t_, Vq, Vp);
-            F32(W, x, q) = __fma_rn(t, Vp, Vq);
-          }
+      if (transf_s) {
+        double c, t, t_;
+        bool trig;
+        if (!SORT) trig = !hyp || pdef;
+        else trig = (pdef != 0);
+        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
+        t_ = trig ? -t : t;
+        if (SORT && trig) {
+          Dp = __fma_rn(t, Apq, Dp);
+          Dq = __fma_rn(t_, Apq, Dq);
+        }
+        bool keep = true;
+        if (SORT)
+          keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
+        const unsigned P = keep ? p : q, Q = keep ? q : p;
+#pragma unroll
+        for (unsigned i = 0u; i < 4u; ++i) {
+          // big rotation: scale by c; otherwise (c == 1) the fma result as is
+          const double rp = __fma_rn(t_, Gq[i], Gp[i]), rq = __fma_rn(t, Gp[i], Gq[i]);
+          const double wp = __fma_rn(t_, Vq[i], Vp[i]), wq = __fma_rn(t, Vp[i], Vq[i]);
+          RA(R, r + 8u * i, P) = transf_b ? c * rp : rp;
+          RA(R, r + 8u * i, Q) = transf_b ? c * rq : rq;
+          RA(W, r + 8u * i, P) = transf_b ? c * wp : wp;
+          RA(W, r + 8u * i, Q) = transf_b ? c * wq : wq;
         }
       }
-      else {
-        if (transf_s) {
-          double c, t, t_;
-          if (pdef) {
-            transf_b = dev_rot_t(Apq, Dp, Dq, c, t);
-            t_ = -t;
-            Dp = __fma_rn(t, Apq, Dp);
-            Dq = __fma_rn(t_, Apq, Dq);
-          }
-          else {
-            transf_b = dev_rot_h(Apq, Dp, Dq, c, t);
-            t_ = t;
-          }
-          const bool keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
-          const unsigned P = keep ? p : q, Q = keep ? q : p;
-          if (transf_b) {
-            F32(R, x, P) = c * __fma_rn(t_, Gq, Gp);
-            F32(R, x, Q) = c * __fma_rn(t, Gp, Gq);
-            F32(W, x, P) = c * __fma_rn(t_, Vq, Vp);
-            F32(W, x, Q) = c * __fma_rn(t, Vp, Vq);
-          }
-          else {
-            F32(R, x, P) = __fma_rn(t_, Gq, Gp);
-            F32(R, x, Q) = __fma_rn(t, Gp, Gq);
-            F32(W, x, P) = __fma_rn(t_, Vq, Vp);
-            F32(W, x, Q) = __fma_rn(t, Vp, Vq);
-          }
-        }
-        else if (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp))) {
-          F32(R, x, p) = Gq;
-          F32(R, x, q) = Gp;
-          F32(W, x, p) = Vq;
-          F32(W, x, q) = Vp;
+      else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
+#pragma unroll
+        for (unsigned i = 0u; i < 4u; ++i) {
+          RA(R, r + 8u * i, p) = Gq[i];
+          RA(R, r + 8u * i, q) = Gp[i];
+          RA(W, r + 8u * i, p) = Vq[i];
+          RA(W, r + 8u * i, q) = Vp[i];
         }
       }
       my_s += (unsigned)transf_s;
       my_b += (unsigned)transf_b;
       __syncthreads();
     }
-    if (x == 0u) {
+    if (r == 0u) {
       cnt_s[y] = my_s;
       cnt_b[y] = my_b;
     }
@@ -252,8 +312,12 @@ __global__ __launch_bounds__(512) void k_pair(const uint16_t *__restrict__ pairs
       break;
   }
 
+  if (r == 0u)
+    cmax[y] = (float)my_cos;
+  __syncthreads();
   // convergence counter, share/device_code_nosort.hpp:94-103
   if (tid == 0u) {
+    unsigned long long *const cvg = &ctl->cvg;
     if (blk_s) {
       if (blk_b)
         atomicAdd(cvg, (((unsigned long long)blk_b) << 32) | blk_s);
@@ -261,12 +325,23 @@ __global__ __launch_bounds__(512) void k_pair(const uint16_t *__restrict__ pairs
         atomicAdd(reinterpret_cast<unsigned *>(cvg), blk_s);
     }
     flags[pi] = SORT ? 1u : (blk_s ? 1u : 0u);
-    pstat[2u * pi] = blk_s;
-    pstat[2u * pi + 1u] = blk_b;
+    float cm = 0.0f;
+    for (unsigned i = 0u; i < 16u; ++i)
+      cm = fmaxf(cm, cmax[i]);
+    VisitRec v;
+    v.rot_s = blk_s;
+    v.rot_b = blk_b;
+    v.maxcos = cm;
+    v.l2ni = bnorm[0];
+    v.l2nj = bnorm[1];
+    v.sweep = ctl->sweep;
+    v.skips = 0u;
+    v.pad = 0u;
+    rec[pi] = v;
   }
   double *const wo = Wout + (size_t)pi * 1024u;
-  wo[tid] = W[tid];
-  wo[tid + 512u] = W[tid + 512u];
+  for (unsigned e = tid; e < 1024u; e += 128u)
+    wo[e] = RA(W, e & 31u, e >> 5);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,11 +395,20 @@ __global__ __launch_bounds__(64) void k_update(double *__restrict__ G, unsigned 
     op0[q] = M + (size_t)(16u * iBlk + cc) * ld + (l & 15u);
     op1[q] = M + (size_t)(16u * jBlk + cc) * ld + (l & 15u);
   }
+  double nb[8];
+#pragma unroll
+  for (unsigned u = 0u; u < 8u; ++u)
+    nb[u] = cp[u][r0];
   for (unsigned r = r0; r < r1; r += 16u) {
     double b[8];
 #pragma unroll
     for (unsigned u = 0u; u < 8u; ++u)
-      b[u] = cp[u][r];
+      b[u] = nb[u];
+    if (r + 16u < r1) {
+#pragma unroll
+      for (unsigned u = 0u; u < 8u; ++u)
+        nb[u] = cp[u][r + 16u];
+    }
     d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
 #pragma unroll
     for (unsigned u = 0u; u < 8u; ++u) {
@@ -384,7 +468,17 @@ __global__ __launch_bounds__(256) void k_initV(double *__restrict__ V, unsigned 
   }
 }
 
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+// Start of a level-2 sweep: keep the previous counter, clear the running
+// one, advance the sweep index (a kernel, so no memset node in the graph).
+__global__ void k_sweep_begin(RunCtl *ctl)
+{
+  if (threadIdx.x == 0u) {
+    ctl->prev_cvg = ctl->cvg;
+    ctl->cvg = 0ull;
+    ctl->sweep += 1u;
+    ctl->skipped = 0u;
+  }
+}
 
 // Scalar self-test kernel.
 __global__ void k_dbg_scalar(int kind, int n, const double *a, const double *b, const double *c,
@@ -406,22 +500,24 @@ __global__ void k_dbg_scalar(int kind, int n, const double *a, const double *b, 
 // Launchers
 
 void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs, unsigned np,
-                 unsigned nchunk, unsigned CH, double *part)
+                 unsigned nchunk, unsigned CH, double *part, const uint8_t *skip)
 {
-  hipLaunchKernelGGL(k_gram, dim3(np * nchunk), dim3(64), 0, s, G, ldG, pairs, nchunk, CH, part);
+  const unsigned nw = gram_waves(CH);
+  hipLaunchKernelGGL(k_gram, dim3(np * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, nchunk, CH, part,
+                     skip);
 }
 
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
-                 unsigned nchunk, double *W, unsigned *flags, unsigned long long *cvg,
-                 unsigned *pstat, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
+                 unsigned nchunk, double *W, unsigned *flags, RunCtl *ctl, VisitRec *rec,
+                 const uint8_t *skip, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
                  int definite, unsigned nplus)
 {
   if (sort)
-    hipLaunchKernelGGL(k_pair<true>, dim3(np), dim3(512), 0, s, pairs, part, nchunk, W, flags, cvg,
-                       pstat, strat0, s0steps, nswp, definite, nplus);
+    hipLaunchKernelGGL(k_pair<true>, dim3(np), dim3(128), 0, s, pairs, part, nchunk, W, flags, ctl,
+                       rec, skip, strat0, s0steps, nswp, definite, nplus);
   else
-    hipLaunchKernelGGL(k_pair<false>, dim3(np), dim3(512), 0, s, pairs, part, nchunk, W, flags,
-                       cvg, pstat, strat0, s0steps, nswp, definite, nplus);
+    hipLaunchKernelGGL(k_pair<false>, dim3(np), dim3(128), 0, s, pairs, part, nchunk, W, flags, ctl,
+                       rec, skip, strat0, s0steps, nswp, definite, nplus);
 }
 
 void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsigned chG,
@@ -450,9 +546,9 @@ void launch_initV(hipStream_t s, double *V, unsigned ifc, unsigned nRank, unsign
   hipLaunchKernelGGL(k_initV, dim3((unsigned)blocks), dim3(256), 0, s, V, ifc, nRank, ldV);
 }
 
-void launch_zero_u64(hipStream_t s, unsigned long long *p)
+void launch_sweep_begin(hipStream_t s, RunCtl *ctl)
 {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
+  hipLaunchKernelGGL(k_sweep_begin, dim3(1), dim3(64), 0, s, ctl);
 }
 
 void launch_dbg_scalar(hipStream_t s, int kind, int n, const double *a, const double *b,

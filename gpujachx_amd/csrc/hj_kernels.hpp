@@ -6,14 +6,52 @@
 #include <cstdint>
 
 #define STRAT0_TAB_BYTES 1024u  // 32 steps x 16 pairs x 2 indices
+#define MAX_GRAM_CHUNKS 16u     // upper bound of gram_nchunk()
+#define GRAM_WAVES 4u           // waves per Gram chunk (upper bound)
 
 namespace hj {
 
+// Per-visit record of one block pair (indexed by step * npairs + slot, the
+// pair's position in the level-1 strategy), written by k_pair at every
+// executed visit and read by the skip predictor.
+struct VisitRec {
+  unsigned rot_s;   // rotations at the last executed visit
+  unsigned rot_b;   // big rotations at the last executed visit
+  float maxcos;     // max |A_pq| / sqrt(A_pp A_qq) over the pivots of its first inner sweep
+  float l2ni;       // log2 ||G_i||_F^2 of the first block (from the Gram)
+  float l2nj;       // log2 ||G_j||_F^2 of the second block
+  unsigned sweep;   // level-2 sweep (1-based) of the last executed visit, 0 = never
+  unsigned skips;   // visits skipped since then
+  unsigned pad;
+};
+
+// Device-side control words of one level-2 run.
+struct RunCtl {
+  unsigned long long cvg;       // rotation counter of the running sweep (big << 32 | all)
+  unsigned long long prev_cvg;  // counter of the previous sweep
+  unsigned sweep;               // running sweep, 1-based
+  unsigned skip_on;             // skipping allowed in this sweep
+  float threshold;              // skip when P(big rotation) < threshold
+  unsigned max_skips;           // at most this many consecutive skipped visits
+  unsigned skipped;             // visits skipped in the running sweep
+  unsigned pad[3];
+};
+
+// waves sharing one Gram chunk of CH rows: each takes CH/nw rows, a multiple
+// of 64 (the wave results are added in wave order)
+inline unsigned gram_waves(unsigned CH)
+{
+  unsigned nw = GRAM_WAVES;
+  while (nw > 1u && (CH % (64u * nw)))
+    nw >>= 1u;
+  return nw;
+}
+
 void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs, unsigned np,
-                 unsigned nchunk, unsigned CH, double *part);
+                 unsigned nchunk, unsigned CH, double *part, const uint8_t *skip);
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
-                 unsigned nchunk, double *W, unsigned *flags, unsigned long long *cvg,
-                 unsigned *pstat, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
+                 unsigned nchunk, double *W, unsigned *flags, RunCtl *ctl, VisitRec *rec,
+                 const uint8_t *skip, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
                  int definite, unsigned nplus);
 void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsigned chG,
                    double *V, unsigned ldV, unsigned nrowV, unsigned chV, const uint16_t *pairs,
@@ -21,7 +59,7 @@ void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsig
 void launch_initD(hipStream_t s, double *G, double *D, unsigned ifc, unsigned nRow, unsigned nRank,
                   unsigned nPlus, unsigned ldG);
 void launch_initV(hipStream_t s, double *V, unsigned ifc, unsigned nRank, unsigned ldV);
-void launch_zero_u64(hipStream_t s, unsigned long long *p);
+void launch_sweep_begin(hipStream_t s, RunCtl *ctl);
 void launch_dbg_scalar(hipStream_t s, int kind, int n, const double *a, const double *b,
                        const double *c, double *oc, double *ot, int *ob);
 

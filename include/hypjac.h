@@ -120,8 +120,53 @@ int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol,
                  unsigned long long *glb_s, unsigned long long *glb_b,
                  double *timing, void *stream);
 
+/* Forms of the method (hj_set_form, bit set; 0 = the reference's method). */
+#define HJ_FORM_LEARNED_SKIP 1u  /* skip visits the pair-skip model predicts
+                                    rotation-free; unskipped certification */
+#define HJ_FORM_SPLIT_GRAM   2u  /* split-precision BF16 MFMA Gram while far
+                                    from convergence, FP64 afterwards */
+void hj_set_form(unsigned form);
+
+/* Parameters of the pair-skip MLP (hj_mlp_nparam() floats, layout W1[F][H],
+ * b1[H], W2[H][H], b2[H], W3[H], b3 with F = hj_mlp_nfeat(), H = 64),
+ * stored as bfloat16; a visit is skipped when the predicted probability of
+ * a big rotation is below `threshold`, at most `max_skips` visits in a row.
+ * Returns -1 on a size mismatch. */
+int hj_set_skip_model(const float *params, unsigned nparam, float threshold,
+                      unsigned max_skips);
+unsigned hj_mlp_nparam(void);
+unsigned hj_mlp_nfeat(void);
+unsigned hj_visit_record_size(void);
+
+/* Visit records (hj_visit_record_size() bytes each, steps*npairs per sweep)
+ * of the following calls are copied to host_buf after every sweep, for up to
+ * max_sweeps sweeps (NULL switches it off): the training traces. */
+void hj_set_visit_trace(void *host_buf, unsigned max_sweeps);
+
+/* Skipped visits per sweep of the last call, their total, executed visits. */
+void hj_last_skips(unsigned *skip, unsigned long long *skipped,
+                   unsigned long long *visits);
+
+/* Features of the next visit computed from records (the same device code
+ * the predictor uses); X: nkeys x hj_mlp_nfeat() floats. */
+int hj_skip_features(const void *rec, unsigned nkeys,
+                     const unsigned short *strat1, unsigned nplus, unsigned n1,
+                     unsigned sweep_now, float prev_frac, float *X);
+
+/* MLP on the GPU (own HIP kernels; BF16 weights, FP32 accumulation):
+ * full-batch loss and gradient (weighted binary cross-entropy, positives
+ * weighted by wpos), prediction, and Adam training (epoch losses out). */
+int hj_mlp_grad(const float *X, const float *y, unsigned n,
+                const float *params, float wpos, float *grads, float *loss);
+int hj_mlp_predict(const float *X, unsigned n, const float *params,
+                   float *prob);
+int hj_mlp_train(const float *X, const float *y, unsigned n, unsigned epochs,
+                 unsigned batch, float lr, float wpos, unsigned seed,
+                 float *params, float *epoch_loss);
+
 /* Per-sweep statistics of the last hypJacL2/hypJacL2_dev call in this
- * process: rot_s[i], rot_b[i] for sweeps i < *nswp (arrays of >= 30). */
+ * process: rot_s[i], rot_b[i] for sweeps i < *nswp (arrays as long as the
+ * sweep cap). */
 void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
                     double *sweep_seconds);
 
@@ -134,6 +179,11 @@ double hj_kernel_ms(int kind, unsigned long long *launches);
 /* Number of row chunks of the Gram of one block pair (for nrow rows and
  * npairs pairs per step); the oracle reproduces the same split. */
 unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs);
+
+/* Level-2 sweep cap (the reference's compile-time HYPJAC_NSWEEP, 30 by
+ * default), 1..256; returns the previous cap.  Inner sweeps (ALG < 8) keep
+ * the reference's cap of 30. */
+unsigned hj_set_max_sweeps(unsigned n);
 
 /* Test limit: stop after `steps` level-1 steps in total (0 = no limit). */
 void hj_set_step_limit(long steps);

@@ -29,6 +29,8 @@ def lib():
         pull = ctypes.POINTER(ctypes.c_ulonglong)
         L.hjo_rsqrt.argtypes = [d]
         L.hjo_rsqrt.restype = d
+        L.hjo_rsqrt_n.argtypes = [pd, pd, ctypes.c_long]
+        L.hjo_rsqrt_n.restype = None
         L.hjo_rot_t.argtypes = [d, d, d, pd, pd]
         L.hjo_rot_t.restype = i
         L.hjo_rot_h.argtypes = [d, d, d, pd, pd]
@@ -48,6 +50,8 @@ def lib():
         L.hjo_hypJacL2.argtypes = [u, u, u, u, pd, u, pd, u, pd, pu, pull, pull,
                                    ctypes.c_void_p, u, ctypes.c_void_p, u, i, ctypes.c_long, pu, pu]
         L.hjo_hypJacL2.restype = i
+        L.hjo_set_max_sweeps.argtypes = [u]
+        L.hjo_set_max_sweeps.restype = u
         _L = L
     return _L
 
@@ -58,6 +62,13 @@ def _pd(a):
 
 def rsqrt(x):
     return lib().hjo_rsqrt(float(x))
+
+
+def rsqrt_n(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().hjo_rsqrt_n(_pd(x), _pd(y), x.size)
+    return y
 
 
 def rot_t(apq, dp, dq):
@@ -98,8 +109,8 @@ def hypJacL2(routine, G, nplus, strat0, strat1, ldhG=None, nthreads=0, max_steps
     s1 = np.ascontiguousarray(strat1, dtype=np.uint16)
     sw = ctypes.c_uint(0)
     gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
-    ts = (ctypes.c_uint * 30)()
-    tb = (ctypes.c_uint * 30)()
+    ts = (ctypes.c_uint * 256)()
+    tb = (ctypes.c_uint * 256)()
     ret = lib().hjo_hypJacL2(routine, nrow, ncol, nplus, _pd(hG), ld, _pd(hV), ncol if full else 0,
                              _pd(hD), ctypes.byref(sw), ctypes.byref(gs), ctypes.byref(gb),
                              s0.ctypes.data, s0.shape[0], s1.ctypes.data, s1.shape[0],
@@ -108,6 +119,10 @@ def hypJacL2(routine, G, nplus, strat0, strat1, ldhG=None, nthreads=0, max_steps
     return dict(ret=ret, G=hG[:, :nrow].T.copy(), V=None if hV is None else hV.T.copy(), D=hD,
                 sweeps=k, rot_s=gs.value, rot_b=gb.value,
                 rot_s_trace=list(ts[:k]), rot_b_trace=list(tb[:k]))
+
+
+def set_max_sweeps(n):
+    return lib().hjo_set_max_sweeps(int(n))
 
 
 def pair(partials, sort, hyp, definite, npos, strat0, nswp=1):

@@ -76,6 +76,12 @@ double hjo_rsqrt(double x)
   return y * u2d((uint64_t)(1023 + s) << 52);
 }
 
+void hjo_rsqrt_n(const double *x, double *y, long n)
+{
+  for (long i = 0; i < n; ++i)
+    y[i] = hjo_rsqrt(x[i]);
+}
+
 /* ---------------------------------------------------------------------- */
 /* share/device_code_common_rotate.hpp:189-213 (dRotT, USE_RSQRT) */
 int hjo_rot_t(double Apq, double Dp, double Dq, double *c, double *t)
@@ -168,10 +174,19 @@ unsigned hjo_gram_nchunk(unsigned nrow, unsigned npairs)
   return nc;
 }
 
-/* Gram of the column pair over a row chunk; reference dGtG,
- * share/device_code_common_Cholesky.hpp:4-56 (only the summation order
- * differs, see hjoracle.h). */
-void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
+/* Waves sharing one chunk of nr rows (a multiple of 64 rows each). */
+unsigned hjo_gram_waves(unsigned nr)
+{
+  unsigned nw = 4u;
+  while (nw > 1u && (nr % (64u * nw)))
+    nw >>= 1u;
+  return nw;
+}
+
+/* Gram of the column pair over rows [r0, r0+nr) by one wave; reference
+ * dGtG, share/device_code_common_Cholesky.hpp:4-56 (only the summation
+ * order differs, see hjoracle.h). */
+static void gram_wave(const double *G0, const double *G1, unsigned ld,
                       unsigned r0, unsigned nr, double out[768])
 {
   double a00[16][16], a10[16][16], a11[16][16];
@@ -179,9 +194,10 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
   memset(a10, 0, sizeof(a10));
   memset(a11, 0, sizeof(a11));
   for (unsigned g = 0u; g < nr / 64u; ++g)
-    for (unsigned t = 0u; t < 16u; ++t)
-      for (unsigned k = 0u; k < 4u; ++k) {
-        const size_t row = (size_t)r0 + 64u * g + 16u * k + t;
+    for (unsigned j = 0u; j < 8u; ++j)
+      for (unsigned e = 0u; e < 2u; ++e)
+        for (unsigned k = 0u; k < 4u; ++k) {
+        const size_t row = (size_t)r0 + 64u * g + 8u * j + 2u * k + e;
         double x0[16], x1[16];
         for (unsigned i = 0u; i < 16u; ++i) {
           x0[i] = G0[(size_t)i * ld + row];
@@ -193,7 +209,7 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
             a10[i][j] = fma(x1[i], x0[j], a10[i][j]);
             a11[i][j] = fma(x1[i], x1[j], a11[i][j]);
           }
-      }
+        }
   /* MFMA D layout: lane l, reg r holds D[4r + l/16][l%16] */
   for (unsigned l = 0u; l < 64u; ++l)
     for (unsigned r = 0u; r < 4u; ++r) {
@@ -202,6 +218,21 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
       out[1 * 256 + l * 4 + r] = a10[i][j];
       out[2 * 256 + l * 4 + r] = a11[i][j];
     }
+}
+
+/* Gram partial of a chunk: hjo_gram_waves(nr) sub-chunks, one per wave,
+ * added in wave order. */
+void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
+                      unsigned r0, unsigned nr, double out[768])
+{
+  const unsigned nw = hjo_gram_waves(nr), sub = nr / nw;
+  gram_wave(G0, G1, ld, r0, sub, out);
+  for (unsigned w = 1u; w < nw; ++w) {
+    double t[768];
+    gram_wave(G0, G1, ld, r0 + w * sub, sub, t);
+    for (unsigned e = 0u; e < 768u; ++e)
+      out[e] = out[e] + t[e];
+  }
 }
 
 #define F32(A, r, c) (A)[(c) * 32u + (r)]
@@ -426,6 +457,15 @@ static void local_npos(unsigned nplus, unsigned iBlk, unsigned jBlk,
   else { *npos = 32u; *definite = +1; }
 }
 
+static unsigned g_max_sweeps = HJO_NSWEEP;
+unsigned hjo_set_max_sweeps(unsigned n)
+{
+  const unsigned old = g_max_sweeps;
+  if (n >= 1u && n <= 256u)
+    g_max_sweeps = n;
+  return old;
+}
+
 /* optional per-visit trace: [sweep][step][pair] -> (rotations, big rotations) */
 static unsigned *g_trace = NULL;
 static size_t g_trace_cap = 0;
@@ -490,7 +530,7 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
   unsigned blk_swp = 0u;
   long steps_done = 0;
   int stop = 0;
-  while (blk_swp < HJO_NSWEEP && !stop) {
+  while (blk_swp < g_max_sweeps && !stop) {
     unsigned long long cvg = 0ull;
     for (unsigned step = 0u; step < s1steps; ++step) {
       if (max_steps > 0 && steps_done >= max_steps) { stop = 1; break; }

@@ -9,10 +9,13 @@
  * Arithmetic order.  Every floating-point operation is an IEEE-754 double
  * operation (fma, +, *, /, sqrt) in the exact order the gfx950 kernels use, so
  * the GPU path is expected to match this oracle bit for bit:
- *   - Gram partial of a row chunk: 64-row groups ascending; inside a group,
- *     16 steps t, each a chain over k=0..3 of fma(x,y,acc) on row
- *     64g+16k+t (the gfx950 FP64 MFMA 16x16x4 is a sequential fma chain in k,
- *     measured on the device);  partials of the chunks are added in order.
+ *   - Gram partial of a row chunk: split into hjo_gram_waves() equal
+ *     sub-chunks, one per wave, whose results are added in wave order; in a
+ *     sub-chunk, 64-row groups ascending; inside a group, 16 MFMA steps
+ *     (j = 0..7, e = 0..1), each a chain over k = 0..3 of fma(x,y,acc) on
+ *     row 64g+8j+2k+e (the gfx950 FP64 MFMA 16x16x4 is a sequential fma
+ *     chain in k, measured on the device); partials of the chunks are added
+ *     in order.
  *   - Block update: out(r,j) = fma chain over k=0..31 of G(r,k)*W(k,j) from +0.
  *   - Column norms in initD: 64 lane-strided fma chains + xor butterfly over
  *     64 lanes (32,16,8,4,2,1).
@@ -49,6 +52,7 @@ extern "C" {
 /* correctly rounded 1/sqrt(x) for positive normal x (own method: IEEE
  * 1/sqrt(x), then a neighbour test with a double-double residual). */
 double hjo_rsqrt(double x);
+void hjo_rsqrt_n(const double *x, double *y, long n);
 
 /* 2x2 rotations, reference share/device_code_common_rotate.hpp:15-64
  * (USE_RSQRT branch).  Return 1 for a "big" rotation. */
@@ -62,6 +66,10 @@ double hjo_sum32(const double v[32]);
 /* number of 64-row-aligned row chunks the Gram of one block pair is split
  * into; identical to the device library's choice for (nrow, pairs). */
 unsigned hjo_gram_nchunk(unsigned nrow, unsigned npairs);
+
+/* Waves sharing one Gram chunk of nr rows: 4, 2 or 1, each taking a
+ * multiple of 64 rows. */
+unsigned hjo_gram_waves(unsigned nr);
 
 /* Gram partial of rows [r0, r0+nr) (nr % 64 == 0) of the column pair
  * (G0: 16 columns of block i, G1: 16 columns of block j, leading dim ld).
@@ -91,8 +99,8 @@ void hjo_initD(double *G, unsigned ld, unsigned nrow, unsigned ncol,
  * argument codes).  strat0: [s0steps][16][2] uint8, strat1: [s1steps][n1/2][2]
  * uint16.  nthreads <= 0: all OpenMP threads.  max_steps > 0 stops after
  * that many level-1 steps in total (bounded CPU-timing sample; the result is
- * then not converged).  rot_s_trace/rot_b_trace (optional, length >= 30)
- * receive the per-sweep ROT_S / ROT_B. */
+ * then not converged).  rot_s_trace/rot_b_trace (optional, as long as the
+ * sweep cap) receive the per-sweep ROT_S / ROT_B. */
 int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
                  double *G, unsigned ldG, double *V, unsigned ldV, double *D,
                  unsigned *glbSwp, unsigned long long *glb_s,
@@ -101,6 +109,9 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
                  const uint16_t *strat1, unsigned s1steps,
                  int nthreads, long max_steps,
                  unsigned *rot_s_trace, unsigned *rot_b_trace);
+
+/* level-2 sweep cap (30 = the reference's HYPJAC_NSWEEP); returns the old */
+unsigned hjo_set_max_sweeps(unsigned n);
 
 /* Per-visit trace for the following hjo_hypJacL2 calls: entry
  * (sweep*s1steps + step)*npairs + pair holds (rotations, big rotations) at

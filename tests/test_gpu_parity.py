@@ -49,9 +49,13 @@ def test_scalar_kernels_bitwise():
     n = 4096
     x = np.concatenate([rng.uniform(0.5, 4.0, n), 10.0 ** rng.uniform(-300, 300, n),
                         1.0 + rng.uniform(0, 1, n) ** 8, np.array([1.0, 2.0, 4.0, 0.25, 5e-324, 1e308])])
+    # plus a million mantissas over [1,4) and values next to powers of 4
+    m = rng.uniform(1.0, 4.0, 1 << 20)
+    edge = np.array([np.nextafter(4.0 ** k, 0.0) for k in range(-3, 4)] +
+                    [np.nextafter(4.0 ** k, np.inf) for k in range(-3, 4)])
+    x = np.concatenate([x, m, m * 2.0 ** -700, edge])
     c, _, _ = g.dbg_scalar(0, x)
-    ref = np.array([o.rsqrt(v) for v in x])
-    np.testing.assert_array_equal(c, ref)
+    np.testing.assert_array_equal(c, o.rsqrt_n(x))
     # rotations across all branches: |ct2| tiny, moderate, huge, around 1
     apq = rng.standard_normal(3 * n) * 10.0 ** rng.uniform(-10, 2, 3 * n)
     dp = 10.0 ** rng.uniform(-3, 3, 3 * n)
