@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
 """Headline benchmark: seconds per SVD of an M=N=8192 FP64 matrix on one MI355X.
 
-Workload (the split-precision configuration's matrix): G = U diag(s) V^T with
-Haar U, V (QR of a seeded N(0,1) matrix, column signs from diag R) and
-s_i = 10^(-12 u_i), u_i ~ U(0,1); strategies cycwor/cycwor; ALG 9 (block
-oriented, sorted, sigma only).  A step is one complete SVD (all sweeps until
-no big rotation, then the column normalisation) of a fresh copy of G that is
-already resident in HBM.
+Workload: G = U diag(s) V^T with Haar U, V (QR of a seeded N(0,1) matrix,
+column signs from diag R) and s_i = 10^(-12 u_i), u_i ~ U(0,1); strategies
+cycwor/cycwor; ALG 9 (block oriented, sorted, sigma only).  A step is one
+complete SVD (all sweeps until an unskipped sweep has no big rotation, then
+the column normalisation) of a fresh copy of G already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--alg A] [--no-cpu]
+`value` is the fast form (learned pair-skip with the committed model); the
+plain FP64 form, bit-identical to the oracle, is timed beside it on the same
+matrix and reported as `plain`, with the ratio.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--alg A] [--form F] [--no-cpu]
 
 Prints one JSON line.  One GPU only: --gpus above 1 exits with a message.
 """
@@ -100,6 +103,13 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=80,
                     help="level-2 sweep cap (the reference's HYPJAC_NSWEEP is 30; this "
                          "matrix needs more at N=8192)")
+    ap.add_argument("--form", choices=["learned+split", "learned", "split", "plain"],
+                    default="learned+split",
+                    help="form timed as `value`; plain is timed beside it unless --no-plain")
+    ap.add_argument("--tau", type=float, default=1e-3,
+                    help="split Gram while a pair's last max |cos| exceeds tau")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--model", default=None, help="skip-model file (default: the packaged one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dgesvj", action="store_true", help="also time LAPACK dgesvj at N=1024")
     args = ap.parse_args()
@@ -124,6 +134,21 @@ def main():
     sw = ctypes.c_uint(0)
     gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
     tm = (ctypes.c_double * 4)()
+    model_info = None
+
+    def set_form(form):
+        nonlocal model_info
+        bits = 0
+        if "learned" in form:
+            import gpujachx_amd.learned as lr
+            _, thr, ms = lr.enable(args.model or lr.DEFAULT_MODEL)
+            model_info = {"file": os.path.relpath(args.model or lr.DEFAULT_MODEL, ROOT),
+                          "threshold": thr, "max_skips": ms}
+            bits |= g.FORM_LEARNED_SKIP
+        if "split" in form:
+            bits |= g.FORM_SPLIT_GRAM
+            g.set_split_tau(args.tau)
+        g.set_form(bits)
 
     def one():
         W.copy_(G0)
@@ -132,46 +157,66 @@ def main():
         if rc:
             raise RuntimeError("hypJacL2_dev returned %d" % rc)
 
-    for _ in range(args.warmup):
-        one()
-    # timed region: kernel durations by HIP events on the work stream
-    g.set_profile(True)
-    kms = np.zeros(3)
-    kcnt = np.zeros(3)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one()
-        for k in range(3):
-            ms, c = g.kernel_ms(k)
-            kms[k] += ms
-            kcnt[k] += c
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    g.set_profile(False)
-    per = elapsed / args.steps
-    sweeps = sw.value
-    rot_s, rot_b, _ = g.last_sweeps()
+    def timed(form, steps, warmup, profile):
+        """Wall time per SVD of `form` over `steps` runs after `warmup`; with
+        profile, per-kernel HIP-event durations over the same runs."""
+        set_form(form)
+        for _ in range(warmup):
+            one()
+        g.set_profile(profile)
+        kms, kcnt = np.zeros(3), np.zeros(3)
+        visits = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            one()
+            for k in range(3):
+                ms, c = g.kernel_ms(k)
+                kms[k] += ms
+                kcnt[k] += c
+            visits += g.last_skips()[2]
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        g.set_profile(False)
+        rot_s, rot_b, _ = g.last_sweeps()
+        skips = g.last_skips()
+        splits = g.last_splits()
+        sig = torch.sqrt(torch.abs(D)).cpu().numpy()
+        return {"per": elapsed / steps, "kms": kms, "kcnt": kcnt, "visits": visits,
+                "sweeps": sw.value, "rot_b": rot_b, "skipped": skips[1], "split": splits[1],
+                "split_per_sweep": splits[0], "skip_per_sweep": skips[0], "sigma": sig}
+
+    other = "plain" if args.form != "plain" else "learned+split"
+    main_run = timed(args.form, args.steps, args.warmup, True)
+    cmp_run = None if args.no_plain else timed(other, 1, 0, True)
+    g.set_form(g.FORM_PLAIN)
+    per = main_run["per"]
+    kms, kcnt = main_run["kms"], main_run["kcnt"]
+    sweeps = main_run["sweeps"]
+    rot_b = main_run["rot_b"]
+    sig = main_run["sigma"]
 
     # accuracy against the constructed spectrum
-    sig = torch.sqrt(torch.abs(D)).cpu().numpy()
     ex = s_exact.cpu().numpy()
     sig_s, ex_s = np.sort(sig)[::-1], np.sort(ex)[::-1]
     rel = np.abs(sig_s - ex_s) / ex_s
     normwise = np.max(np.abs(sig_s - ex_s)) / ex_s[0]
 
-    # roofline of the dominant kernel (the block update) and of the Gram
+    # roofline of the dominant kernel (the block update) and of the Gram:
+    # algorithmic bytes per launch = executed pair visits per launch x bytes
+    # per visit (read + write of the m x 32 pair: 512 m; Gram: read, 256 m)
     np_ = n // 32
     steps_per_sweep = n // 16 - 1
-    upd_bytes = np_ * 512.0 * n          # read + write of the m x 32 pair, per launch
-    gram_bytes = np_ * 256.0 * n         # read of the m x 32 pair, per launch
+    pairs_per_launch = main_run["visits"] / max(kcnt[2], 1)
+    upd_bytes = pairs_per_launch * 512.0 * n
+    gram_bytes = pairs_per_launch * 256.0 * n
     upd_ms = kms[2] / max(kcnt[2], 1)
     gram_ms = kms[0] / max(kcnt[0], 1)
     pair_ms = kms[1] / max(kcnt[1], 1)
     ach = upd_bytes / (upd_ms * 1e-3) / 1e9
     gram_ach = gram_bytes / (gram_ms * 1e-3) / 1e9
-    upd_tf = np_ * 2048.0 * n / (upd_ms * 1e-3) / 1e12
-    gram_tf = np_ * 1536.0 * n / (gram_ms * 1e-3) / 1e12
+    upd_tf = pairs_per_launch * 2048.0 * n / (upd_ms * 1e-3) / 1e12
+    gram_tf = pairs_per_launch * 1536.0 * n / (gram_ms * 1e-3) / 1e12
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "update_traffic.json")
@@ -192,14 +237,20 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if "split" not in args.form else "f64 (split Gram: bf16x3 MFMA, fp32 acc)",
         "data": "synthetic",
         "config": {"workload": "SVD M=N=%d, G=U diag(10^-12u) V^T (Haar), ALG %d, cycwor/cycwor, "
-                               "plain FP64 form" % (n, alg),
-                   "M": n, "N": n, "alg": alg, "form": "plain-fp64", "seed": args.seed,
-                   "max_sweeps": args.max_sweeps},
+                               "%s form" % (n, alg, args.form),
+                   "M": n, "N": n, "alg": alg, "form": args.form, "seed": args.seed,
+                   "max_sweeps": args.max_sweeps, "tau": args.tau if "split" in args.form else None,
+                   "skip_model": model_info if "learned" in args.form else None},
         "converged": bool(rot_b and rot_b[-1] == 0),
         "sweeps": sweeps,
+        "visits_executed": main_run["visits"] // max(args.steps, 1),
+        "visits_skipped": main_run["skipped"],
+        "visits_split_gram": main_run["split"],
+        "skip_per_sweep": main_run["skip_per_sweep"],
+        "split_per_sweep": main_run["split_per_sweep"],
         "rot_b_per_sweep": rot_b,
         "sigma_err": {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
                       "normwise": float(normwise),
@@ -214,6 +265,15 @@ def main():
                     "update": {"avg_ms": upd_ms, "GBs": ach, "tflops": upd_tf},
                     "launches_per_step": float(kcnt[2] / args.steps)},
     }
+    if cmp_run is not None:
+        cs = np.sort(cmp_run["sigma"])[::-1]
+        out[other] = {"s_per_svd": cmp_run["per"], "sweeps": cmp_run["sweeps"],
+                      "visits_executed": cmp_run["visits"], "visits_skipped": cmp_run["skipped"],
+                      "sigma_normwise_diff_vs_value": float(np.max(np.abs(cs - sig_s)) / sig_s[0])}
+        out["speedup_vs_plain"] = (cmp_run["per"] / per) if other == "plain" else (per / cmp_run["per"])
+        cmp_rel = np.abs(cs - ex_s) / ex_s
+        out[other]["sigma_err"] = {"max_rel": float(cmp_rel.max()),
+                                   "normwise": float(np.max(np.abs(cs - ex_s)) / ex_s[0])}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(G0.T.cpu().numpy().copy(), n, alg, sweeps,
                                            steps_per_sweep)

@@ -94,6 +94,12 @@ def lib():
     L.hj_mlp_predict.restype = i
     L.hj_mlp_train.argtypes = [pf, pf, u, u, u, ctypes.c_float, ctypes.c_float, u, pf, pf]
     L.hj_mlp_train.restype = i
+    L.hj_set_split_tau.argtypes = [ctypes.c_float]
+    L.hj_set_split_tau.restype = None
+    L.hj_last_splits.argtypes = [pu, pull]
+    L.hj_last_splits.restype = None
+    L.hj_dbg_gram.argtypes = [i, u, pd, pd]
+    L.hj_dbg_gram.restype = i
     _lib = L
     return L
 
@@ -216,11 +222,34 @@ def set_form(form):
 
 def last_skips():
     """(per-sweep skipped visits, total skipped, total executed visits)."""
-    n, _, _ = len(last_sweeps()[0]), None, None
+    n = len(last_sweeps()[0])
     arr = (ctypes.c_uint * 256)()
     sk, vi = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
     lib().hj_last_skips(arr, ctypes.byref(sk), ctypes.byref(vi))
     return list(arr[:n]), sk.value, vi.value
+
+
+def set_split_tau(tau):
+    lib().hj_set_split_tau(float(tau))
+
+
+def last_splits():
+    """(per-sweep split-Gram visits, total) of the last call."""
+    n = len(last_sweeps()[0])
+    arr = (ctypes.c_uint * 256)()
+    tot = ctypes.c_ulonglong(0)
+    lib().hj_last_splits(arr, ctypes.byref(tot))
+    return list(arr[:n]), tot.value
+
+
+def dbg_gram(G, split):
+    """32x32 Gram of an nrow x 32 matrix from the FP64 or the split kernel."""
+    G = np.asfortranarray(G, dtype=np.float64)
+    assert G.shape[1] == 32
+    A = np.zeros((32, 32), dtype=np.float64, order="F")
+    if lib().hj_dbg_gram(1 if split else 0, G.shape[0], _pd(G), _pd(A)):
+        raise ValueError("hj_dbg_gram: nrow must be a positive multiple of 64")
+    return np.ascontiguousarray(A)
 
 
 def set_profile(on):

@@ -25,9 +25,10 @@ struct Stats {
   unsigned rot_b[HJ_MAX_SWEEP_CAP] = {0u};
   double sec[HJ_MAX_SWEEP_CAP] = {0.0};
   unsigned skip[HJ_MAX_SWEEP_CAP] = {0u};
+  unsigned split[HJ_MAX_SWEEP_CAP] = {0u};
   double kms[3] = {0.0, 0.0, 0.0};
   unsigned long long kcnt[3] = {0ull, 0ull, 0ull};
-  unsigned long long skipped = 0ull, visits = 0ull;
+  unsigned long long skipped = 0ull, visits = 0ull, splits = 0ull;
 };
 static Stats g_stats;
 static int g_profile = 0;
@@ -39,6 +40,7 @@ static bool g_model_ready = false;
 static uint16_t g_model_bf16[MLP_NPARAM];
 static float g_skip_threshold = 0.05f;
 static unsigned g_skip_max = 2u;
+static float g_split_tau = 1e-3f;  // split Gram while the pair's last max |cos| > tau
 // per-sweep visit records copied to the host (trace collection)
 static void *g_trace_buf = nullptr;
 static unsigned g_trace_sweeps = 0u;
@@ -153,13 +155,16 @@ static void check_tables(unsigned ncol)
 // One level-1 step: Gram partials, pair kernels, block update.
 static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, unsigned nswp,
                        int definite, unsigned nplus, double *dG, unsigned lddG, unsigned nrow,
-                       double *dV, unsigned lddV, unsigned ncol, hipEvent_t *ev, bool skipping)
+                       double *dV, unsigned lddV, unsigned ncol, hipEvent_t *ev, bool planned,
+                       bool split)
 {
   const size_t off = (size_t)step * w.np;
   const uint16_t *pairs = w.d_s1 + off * 2u;
-  const uint8_t *skip = skipping ? w.d_skip + off : nullptr;
+  const uint8_t *skip = planned ? w.d_skip + off : nullptr;
   if (ev) HIP_CALL(hipEventRecord(ev[0], s));
   launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
+  if (split)
+    launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
   if (ev) HIP_CALL(hipEventRecord(ev[1], s));
   launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
               skip, w.d_s0, w.s0steps, nswp, definite, nplus);
@@ -204,6 +209,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
   double *const V = full_svd ? dV : nullptr;
   const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready;
+  const bool split = (g_form & HJ_FORM_SPLIT_GRAM) != 0u;
+  const bool planned = learned || split;
   if (learned)
     HIP_CALL(hipMemcpy(w.d_model, g_model_bf16, sizeof(uint16_t) * MLP_NPARAM, hipMemcpyHostToDevice));
   const float log2n1 = (float)std::log2((double)(ncol / 16u));
@@ -221,9 +228,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   }
   auto sweep_head = [&](hipStream_t st) {
     launch_sweep_begin(st, w.d_ctl);
-    if (learned)
-      launch_skip_infer(st, w.d_rec, (unsigned)w.keys(), w.d_s1, nplus, log2n1, max_rot, w.d_ctl,
-                        w.d_model, w.d_skip);
+    if (planned)
+      launch_plan(st, w.d_rec, (unsigned)w.keys(), w.d_s1, nplus, log2n1, max_rot, w.d_ctl, w.d_model,
+                  learned, w.d_skip);
   };
   if (!eager) {
     // one sweep as a graph: start-of-sweep kernel (counters, no memset
@@ -233,7 +240,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     sweep_head(s);
     for (unsigned st = 0u; st < w.steps; ++st)
       issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
-                 learned);
+                 planned, split);
     HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamEndCapture(s, &graph));
     HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -247,14 +254,20 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   bool stop = false, certify = false;
   g_stats.skipped = 0ull;
   while (blk_swp < g_max_sweeps) {
-    if (learned) {
-      // skipping on unless this is the first or a certification sweep
-      RunCtl c;
-      c.skip_on = (blk_swp > 0u && !certify) ? 1u : 0u;
+    if (planned) {
+      // skipping on unless this is the first or a certification sweep; the
+      // split Gram on unless this is a certification sweep
+      RunCtl &c = *w.h_ctl;
+      c.skip_on = (learned && blk_swp > 0u && !certify) ? 1u : 0u;
       c.threshold = g_skip_threshold;
       c.max_skips = g_skip_max;
+      c.split_on = (split && !certify) ? 1u : 0u;
+      c.tau = g_split_tau;
       HIP_CALL(hipMemcpyAsync(reinterpret_cast<char *>(w.d_ctl) + offsetof(RunCtl, skip_on), &c.skip_on,
                               offsetof(RunCtl, skipped) - offsetof(RunCtl, skip_on),
+                              hipMemcpyHostToDevice, s));
+      HIP_CALL(hipMemcpyAsync(reinterpret_cast<char *>(w.d_ctl) + offsetof(RunCtl, split_on), &c.split_on,
+                              offsetof(RunCtl, split) - offsetof(RunCtl, split_on),
                               hipMemcpyHostToDevice, s));
       HIP_CALL(hipStreamSynchronize(s));
     }
@@ -267,7 +280,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
         }
         ++steps_done;
         issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
-                   g_profile ? &evs[(size_t)4u * st] : nullptr, learned);
+                   g_profile ? &evs[(size_t)4u * st] : nullptr, planned, split);
       }
       HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     }
@@ -285,6 +298,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     ++blk_swp;
     const unsigned long long cvg = w.h_ctl->cvg;
     const unsigned skipped = w.h_ctl->skipped;
+    const unsigned nsplit = w.h_ctl->split;
     const unsigned cvg_s = static_cast<unsigned>(cvg);
     const unsigned cvg_b = static_cast<unsigned>(cvg >> 32u);
     *glb_s += cvg_s;
@@ -295,7 +309,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       g_stats.rot_b[blk_swp - 1u] = cvg_b;
       g_stats.sec[blk_swp - 1u] = tim_s;
       g_stats.skip[blk_swp - 1u] = skipped;
+      g_stats.split[blk_swp - 1u] = nsplit;
     }
+    g_stats.splits += nsplit;
     g_stats.nswp = blk_swp;
     g_stats.skipped += skipped;
     g_stats.visits += w.keys() - skipped;
@@ -311,8 +327,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     if (stop)
       break;
     if (!cvg_b) {
-      if (skipped) {
-        certify = true;  // the next sweep runs every visit
+      if (skipped || nsplit) {
+        certify = true;  // the next sweep runs every visit, all in FP64
         continue;
       }
       break;
@@ -378,7 +394,7 @@ static void thin_step(unsigned step, bool sort, bool withV, int definite, unsign
   if (!g_thin.ready) HJ_DIE("initSymbols not called");
   if (step >= g_thin.w.steps) HJ_DIE("step out of range");
   issue_step(nullptr, g_thin.w, step, sort, g_thin.nSwp, definite, nplus, g_thin.G, g_thin.ldG,
-             g_thin.nRow, withV ? g_thin.V : nullptr, g_thin.ldV, g_thin.nCol, nullptr, false);
+             g_thin.nRow, withV ? g_thin.V : nullptr, g_thin.ldV, g_thin.nCol, nullptr, false, false);
   hipLaunchKernelGGL(k_thin_accum, dim3(1), dim3(64), 0, nullptr, g_thin.w.d_ctl, g_thin.cvg);
   HIP_CALL(hipGetLastError());
 }
@@ -506,6 +522,60 @@ extern "C" void hj_last_skips(unsigned *skip, unsigned long long *skipped, unsig
     if (skip) skip[i] = g_stats.skip[i];
   if (skipped) *skipped = g_stats.skipped;
   if (visits) *visits = g_stats.visits;
+}
+
+extern "C" void hj_set_split_tau(float tau) { g_split_tau = tau; }
+
+extern "C" void hj_last_splits(unsigned *split, unsigned long long *total)
+{
+  for (unsigned i = 0u; i < g_stats.nswp && i < HJ_MAX_SWEEP_CAP; ++i)
+    if (split) split[i] = g_stats.split[i];
+  if (total) *total = g_stats.splits;
+}
+
+// The 32x32 Gram of one 32-column block pair (columns 0..31 of a column-major
+// nrow x 32 host matrix, leading dimension nrow) from the FP64 (split = 0) or
+// the split-precision (split = 1) Gram kernel, partials summed in chunk
+// order; both triangles are filled.
+extern "C" int hj_dbg_gram(int split, unsigned nrow, const double *hG, double *hA)
+{
+  if (!hG || !hA || !nrow || (nrow % 64u)) return -1;
+  const unsigned nchunk = gram_nchunk(nrow, 1u), CH = nrow / nchunk;
+  double *dG = nullptr, *dP = nullptr;
+  uint16_t *dpairs = nullptr;
+  uint8_t *dplan = nullptr;
+  HIP_CALL(hipMalloc(&dG, sizeof(double) * 32u * nrow));
+  HIP_CALL(hipMalloc(&dP, sizeof(double) * 768u * nchunk));
+  HIP_CALL(hipMalloc(&dpairs, 2u * sizeof(uint16_t)));
+  HIP_CALL(hipMalloc(&dplan, 1u));
+  const uint16_t pr[2] = {0u, 1u};
+  const uint8_t pl = split ? PLAN_SPLIT : PLAN_FP64;
+  HIP_CALL(hipMemcpy(dG, hG, sizeof(double) * 32u * nrow, hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(dpairs, pr, sizeof(pr), hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(dplan, &pl, 1u, hipMemcpyHostToDevice));
+  if (split)
+    launch_gram_split(nullptr, dG, nrow, dpairs, 1u, nchunk, CH, dP, dplan);
+  else
+    launch_gram(nullptr, dG, nrow, dpairs, 1u, nchunk, CH, dP, dplan);
+  HIP_CALL(hipGetLastError());
+  HIP_CALL(hipDeviceSynchronize());
+  std::vector<double> P((size_t)768u * nchunk);
+  HIP_CALL(hipMemcpy(P.data(), dP, sizeof(double) * P.size(), hipMemcpyDeviceToHost));
+  for (unsigned e = 0u; e < 768u; ++e) {
+    double v = P[e];
+    for (unsigned c = 1u; c < nchunk; ++c) v = v + P[(size_t)c * 768u + e];
+    const unsigned tile = e >> 8, l = (e >> 2) & 63u, r = e & 3u;
+    const unsigned i = 4u * r + (l >> 4), j = l & 15u;
+    const unsigned row = (tile == 0u) ? i : (16u + i);
+    const unsigned col = (tile == 2u) ? (16u + j) : j;
+    hA[(size_t)col * 32u + row] = v;
+    if (tile == 1u) hA[(size_t)row * 32u + col] = v;
+  }
+  HIP_CALL(hipFree(dG));
+  HIP_CALL(hipFree(dP));
+  HIP_CALL(hipFree(dpairs));
+  HIP_CALL(hipFree(dplan));
+  return 0;
 }
 
 extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)

@@ -17,6 +17,11 @@
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Output rows of the 16x16 MFMA results (lane l, register r; the column is
+// l % 16), measured on gfx950: FP64 16x16x4 holds row 4r + l/16; FP32
+// 16x16x32 BF16 holds row 4(l/16) + r.
+#define F32_OUT_ROW(l, r) (4u * ((l) >> 4) + (r))
+
 namespace hj {
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
@@ -103,6 +108,190 @@ __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsi
 }
 
 // ---------------------------------------------------------------------------
+// Split-precision Gram partials (same grid, chunks, waves and output layout
+// as k_gram) for the pairs whose plan byte is PLAN_SPLIT.
+//
+// Per 64-row group a wave scales each block by a power of two (2^-e with
+// 2^(e-1) <= max |x| < 2^e over the group, so the scaled values lie in
+// (-1, 1) whatever the FP64 range), rounds to FP32 and splits every value
+// into three bfloat16 pieces x = h + m + l (h = bf16(x), m = bf16(x - h),
+// l = bf16(x - h - m): the FP32 value exactly, up to the last piece's
+// rounding).  The products of a tile are summed on BF16 MFMA 16x16x32 with
+// FP32 accumulation in three magnitude classes, hh, hm + mh, hl + lh + mm
+// (the dropped terms ml, lm, ll are below 2^-24 relative), and after each
+// group the classes are added smallest first in FP64, unscaled exactly by
+// 2^(e_a + e_b) and added to the FP64 accumulators.  Relative error of an
+// entry, against sqrt(A_ii A_jj): about 2^-22 per group, FP64 across groups.
+//
+// Rows: lane l (column l%16, lane group q = l/16) holds rows 16q .. 16q+15 of
+// the group, 128 contiguous bytes per column; K-block 0 takes the first 8
+// (MFMA k = 8q + t), K-block 1 the last 8, so every row is taken once.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+typedef __attribute__((ext_vector_type(4))) float f4;
+
+// row of a 16x16 FP32 MFMA result held by lane l in register r
+__device__ __forceinline__ unsigned f32_out_row(unsigned l, unsigned r)
+{
+  return F32_OUT_ROW(l, r);
+}
+
+__device__ __forceinline__ f4 mfma_bf(const bf8 &a, const bf8 &b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// biased FP64 exponent field of x (0 for zero / subnormal)
+__device__ __forceinline__ int exp_field(double x)
+{
+  return (__double2hiint(x) >> 20) & 0x7ff;
+}
+
+__device__ __forceinline__ int wave_max_int(int v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// three-way bf16 split of 8 FP64 values scaled by 2^-e
+__device__ __forceinline__ void split3(const double *x, int e, bf8 &h, bf8 &m, bf8 &lo)
+{
+#pragma unroll
+  for (unsigned t = 0u; t < 8u; ++t) {
+    const float v = (float)ldexp(x[t], -e);
+    const __bf16 a = (__bf16)v;
+    const float r1 = v - (float)a;
+    const __bf16 b = (__bf16)r1;
+    const float r2 = r1 - (float)b;
+    h[t] = a;
+    m[t] = b;
+    lo[t] = (__bf16)r2;
+  }
+}
+
+struct Cls {
+  f4 c0, c1, c2;  // hh, hm + mh, hl + lh + mm
+};
+
+__device__ __forceinline__ void tile_products(Cls &C, const bf8 &ha, const bf8 &ma, const bf8 &la,
+                                              const bf8 &hb, const bf8 &mb, const bf8 &lb)
+{
+  C.c0 = mfma_bf(ha, hb, C.c0);
+  C.c1 = mfma_bf(ha, mb, C.c1);
+  C.c1 = mfma_bf(ma, hb, C.c1);
+  C.c2 = mfma_bf(ha, lb, C.c2);
+  C.c2 = mfma_bf(la, hb, C.c2);
+  C.c2 = mfma_bf(ma, mb, C.c2);
+}
+
+__device__ __forceinline__ void fold(d4 &acc, Cls &C, int e)
+{
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    double t = (double)C.c2[r] + (double)C.c1[r];
+    t = t + (double)C.c0[r];
+    acc[r] = acc[r] + ldexp(t, e);
+    C.c0[r] = 0.0f;
+    C.c1[r] = 0.0f;
+    C.c2[r] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gram_split(const double *__restrict__ G, unsigned ldG,
+                                                    const uint16_t *__restrict__ pairs,
+                                                    unsigned nchunk, unsigned CH,
+                                                    double *__restrict__ part,
+                                                    const uint8_t *__restrict__ plan)
+{
+  __shared__ d4 red[3][GRAM_WAVES - 1][64];
+  const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned pi = blockIdx.x / nchunk, c = blockIdx.x - pi * nchunk;
+  if (plan[pi] != PLAN_SPLIT)
+    return;
+  const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
+  const unsigned SUB = CH / nw;
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 16u * (l >> 4);
+  const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
+  const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
+  d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
+  Cls C00, C10, C11;
+  C00.c0 = C00.c1 = C00.c2 = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  C10 = C00;
+  C11 = C00;
+  double2 n0[8], n1[8];
+#pragma unroll
+  for (unsigned j = 0u; j < 8u; ++j) {
+    n0[j] = *reinterpret_cast<const double2 *>(g0 + 2u * j);
+    n1[j] = *reinterpret_cast<const double2 *>(g1 + 2u * j);
+  }
+  for (unsigned g = 0u; g < SUB; g += 64u) {
+    double x0[16], x1[16];
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      x0[2u * j] = n0[j].x;
+      x0[2u * j + 1u] = n0[j].y;
+      x1[2u * j] = n1[j].x;
+      x1[2u * j + 1u] = n1[j].y;
+    }
+    if (g + 64u < SUB) {
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j) {
+        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 64u + 2u * j);
+        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 64u + 2u * j);
+      }
+    }
+    int e0 = 0, e1 = 0;
+#pragma unroll
+    for (unsigned t = 0u; t < 16u; ++t) {
+      e0 = max(e0, exp_field(x0[t]));
+      e1 = max(e1, exp_field(x1[t]));
+    }
+    e0 = wave_max_int(e0);
+    e1 = wave_max_int(e1);
+    e0 = e0 ? (e0 - 1022) : 0;
+    e1 = e1 ? (e1 - 1022) : 0;
+#pragma unroll
+    for (unsigned kb = 0u; kb < 2u; ++kb) {
+      bf8 h0, m0, q0, h1, m1, q1;
+      split3(x0 + 8u * kb, e0, h0, m0, q0);
+      split3(x1 + 8u * kb, e1, h1, m1, q1);
+      tile_products(C00, h0, m0, q0, h0, m0, q0);
+      tile_products(C10, h1, m1, q1, h0, m0, q0);
+      tile_products(C11, h1, m1, q1, h1, m1, q1);
+    }
+    fold(a00, C00, e0 + e0);
+    fold(a10, C10, e1 + e0);
+    fold(a11, C11, e1 + e1);
+  }
+  if (w) {
+    red[0][w - 1u][l] = a00;
+    red[1][w - 1u][l] = a10;
+    red[2][w - 1u][l] = a11;
+  }
+  __syncthreads();
+  if (w == 0u) {
+    for (unsigned v = 1u; v < nw; ++v) {
+      a00 = a00 + red[0][v - 1u][l];
+      a10 = a10 + red[1][v - 1u][l];
+      a11 = a11 + red[2][v - 1u][l];
+    }
+    // to the FP64 MFMA layout of the partials: element (i, j) at lane
+    // j + 16 (i % 4), register i / 4
+    double *const o = part + (size_t)blockIdx.x * 768u;
+    const unsigned j = l & 15u;
+#pragma unroll
+    for (unsigned r = 0u; r < 4u; ++r) {
+      const unsigned i = f32_out_row(l, r);
+      const unsigned idx = (j + 16u * (i & 3u)) * 4u + (i >> 2);
+      o[idx] = a00[r];
+      o[256u + idx] = a10[r];
+      o[512u + idx] = a11[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Block pair: Gram -> Cholesky -> inner Jacobi -> W, two waves per pair.
 // R and W live in LDS, column-major with a padded leading dimension.
 // Cholesky: wave 0, lane x (x < 32) keeps row x of the Gram in registers;
@@ -129,10 +318,12 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
   __shared__ unsigned cnt_s[16], cnt_b[16];
   __shared__ float cmax[16];
   __shared__ float bnorm[2];
+  __shared__ int chol_ok;
 
   const unsigned tid = threadIdx.x;
   const unsigned pi = blockIdx.x;
-  if (skip && skip[pi]) {
+  const uint8_t plan = skip ? skip[pi] : PLAN_FP64;
+  if (plan == PLAN_SKIP) {
     // skipped visit: no update, the record keeps the last executed visit
     if (tid == 0u) {
       flags[pi] = 0u;
@@ -215,8 +406,37 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
       for (unsigned y = 0u; y < 32u; ++y)
         RA(R, y, x) = (y <= x) ? a[y] : 0.0;
     }
+    if (plan == PLAN_SPLIT) {
+      // a Gram from the split products may miss positive definiteness
+      // (nearly dependent columns): then no update, FP64 at the next visit
+      bool ok = true;
+#pragma unroll
+      for (unsigned j = 0u; j < 32u; ++j) {
+        ok = ok && (fabs(a[j]) <= 0x1.fffffffffffffp+1023);
+        if (j == x) ok = ok && (a[j] > 0.0);
+      }
+      const unsigned long long good = __ballot(ok);
+      if (tid == 0u) chol_ok = (good == ~0ull) ? 1 : 0;
+    }
   }
   __syncthreads();
+  if (plan == PLAN_SPLIT && !chol_ok) {
+    if (tid == 0u) {
+      flags[pi] = 0u;
+      VisitRec v;
+      v.rot_s = 0u;
+      v.rot_b = 0u;
+      v.maxcos = 1.0f;
+      v.l2ni = bnorm[0];
+      v.l2nj = bnorm[1];
+      v.sweep = ctl->sweep;
+      v.skips = 0u;
+      v.flags = REC_SPLIT | REC_SPLIT_FAIL;
+      rec[pi] = v;
+      atomicAdd(&ctl->split, 1u);
+    }
+    return;
+  }
 
   // Inner Jacobi: share/device_code_nosort.hpp:4-222 (SORT false),
   // share/device_code_cdsort_KISS.hpp:4-260 and share/device_code_cdsort.hpp:25-167
@@ -336,8 +556,10 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
     v.l2nj = bnorm[1];
     v.sweep = ctl->sweep;
     v.skips = 0u;
-    v.pad = 0u;
+    v.flags = (plan == PLAN_SPLIT) ? REC_SPLIT : 0u;
     rec[pi] = v;
+    if (plan == PLAN_SPLIT)
+      atomicAdd(&ctl->split, 1u);
   }
   double *const wo = Wout + (size_t)pi * 1024u;
   for (unsigned e = tid; e < 1024u; e += 128u)
@@ -477,6 +699,7 @@ __global__ void k_sweep_begin(RunCtl *ctl)
     ctl->cvg = 0ull;
     ctl->sweep += 1u;
     ctl->skipped = 0u;
+    ctl->split = 0u;
   }
 }
 
@@ -505,6 +728,14 @@ void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *p
   const unsigned nw = gram_waves(CH);
   hipLaunchKernelGGL(k_gram, dim3(np * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, nchunk, CH, part,
                      skip);
+}
+
+void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
+                       unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan)
+{
+  const unsigned nw = gram_waves(CH);
+  hipLaunchKernelGGL(k_gram_split, dim3(np * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, nchunk, CH,
+                     part, plan);
 }
 
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,

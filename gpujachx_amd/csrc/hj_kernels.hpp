@@ -22,8 +22,17 @@ struct VisitRec {
   float l2nj;       // log2 ||G_j||_F^2 of the second block
   unsigned sweep;   // level-2 sweep (1-based) of the last executed visit, 0 = never
   unsigned skips;   // visits skipped since then
-  unsigned pad;
+  unsigned flags;   // REC_* bits of the last executed visit
 };
+
+// VisitRec.flags
+constexpr unsigned REC_SPLIT = 1u;       // the visit used the split-precision Gram
+constexpr unsigned REC_SPLIT_FAIL = 2u;  // ... whose Cholesky failed: no update, FP64 next
+
+// visit plan of the running sweep (one byte per (step, pair))
+constexpr uint8_t PLAN_FP64 = 0u;   // FP64 Gram
+constexpr uint8_t PLAN_SKIP = 1u;   // skipped
+constexpr uint8_t PLAN_SPLIT = 2u;  // split-precision Gram
 
 // Device-side control words of one level-2 run.
 struct RunCtl {
@@ -34,7 +43,9 @@ struct RunCtl {
   float threshold;              // skip when P(big rotation) < threshold
   unsigned max_skips;           // at most this many consecutive skipped visits
   unsigned skipped;             // visits skipped in the running sweep
-  unsigned pad[3];
+  unsigned split_on;            // split-precision Gram allowed in this sweep
+  float tau;                    // ... for pairs whose last max |cos| exceeds tau
+  unsigned split;               // visits with the split Gram in the running sweep
 };
 
 // waves sharing one Gram chunk of CH rows: each takes CH/nw rows, a multiple
@@ -49,6 +60,8 @@ inline unsigned gram_waves(unsigned CH)
 
 void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs, unsigned np,
                  unsigned nchunk, unsigned CH, double *part, const uint8_t *skip);
+void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
+                       unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
                  unsigned nchunk, double *W, unsigned *flags, RunCtl *ctl, VisitRec *rec,
                  const uint8_t *skip, const uint8_t *strat0, unsigned s0steps, unsigned nswp,

@@ -97,33 +97,44 @@ __device__ float mlp_logit(const float *w, const float *f)
   return z;
 }
 
-// Skip mask of the running sweep: one thread per (step, slot).
-__global__ __launch_bounds__(256) void k_skip_infer(const VisitRec *__restrict__ rec, unsigned nkeys,
-                                                    const uint16_t *__restrict__ strat1, unsigned nplus,
-                                                    float log2n1, float max_rot,
-                                                    const RunCtl *__restrict__ ctl,
-                                                    const uint16_t *__restrict__ model,
-                                                    uint8_t *__restrict__ skip)
+// Plan of the running sweep, one thread per (step, slot): PLAN_SKIP when
+// skipping is on and the model's probability of a big rotation is below the
+// threshold (at most max_skips visits in a row); otherwise PLAN_SPLIT when
+// the split Gram is on and the pair is far from convergence (never visited,
+// or its last max |cos| above tau and its last split Cholesky did not fail);
+// otherwise PLAN_FP64.
+__global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, unsigned nkeys,
+                                              const uint16_t *__restrict__ strat1, unsigned nplus,
+                                              float log2n1, float max_rot,
+                                              const RunCtl *__restrict__ ctl,
+                                              const uint16_t *__restrict__ model, int learned,
+                                              uint8_t *__restrict__ plan)
 {
   __shared__ float w[MLP_NPARAM];
-  for (unsigned e = threadIdx.x; e < MLP_NPARAM; e += blockDim.x)
-    w[e] = bf16_to_float(model[e]);
-  __syncthreads();
+  if (learned) {
+    for (unsigned e = threadIdx.x; e < MLP_NPARAM; e += blockDim.x)
+      w[e] = bf16_to_float(model[e]);
+    __syncthreads();
+  }
   const unsigned key = blockIdx.x * blockDim.x + threadIdx.x;
   if (key >= nkeys)
     return;
   const VisitRec v = rec[key];
-  uint8_t s = 0u;
-  if (ctl->skip_on && v.sweep != 0u && v.skips < ctl->max_skips) {
+  uint8_t s = PLAN_FP64;
+  if (learned && ctl->skip_on && v.sweep != 0u && v.skips < ctl->max_skips) {
     const float prev = (float)(unsigned)(ctl->prev_cvg >> 32) / max_rot;
     float f[MLP_F];
     visit_features(v, ctl->sweep, prev, pair_jmix(strat1[2u * key], strat1[2u * key + 1u], nplus),
                    log2n1, f);
     const float z = mlp_logit(w, f);
     const float p = 1.0f / (1.0f + expf(-z));
-    s = (p < ctl->threshold) ? 1u : 0u;
+    if (p < ctl->threshold)
+      s = PLAN_SKIP;
   }
-  skip[key] = s;
+  if (s == PLAN_FP64 && ctl->split_on &&
+      (v.sweep == 0u || (v.maxcos > ctl->tau && !(v.flags & REC_SPLIT_FAIL))))
+    s = PLAN_SPLIT;
+  plan[key] = s;
 }
 
 // Features of many records (training-set construction).
@@ -234,12 +245,12 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__re
   p[i] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-void launch_skip_infer(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
-                       unsigned nplus, float log2n1, float max_rot, const RunCtl *ctl,
-                       const uint16_t *model, uint8_t *skip)
+void launch_plan(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
+                 unsigned nplus, float log2n1, float max_rot, const RunCtl *ctl,
+                 const uint16_t *model, bool learned, uint8_t *plan)
 {
-  hipLaunchKernelGGL(k_skip_infer, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, rec, nkeys, strat1,
-                     nplus, log2n1, max_rot, ctl, model, skip);
+  hipLaunchKernelGGL(k_plan, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, rec, nkeys, strat1, nplus,
+                     log2n1, max_rot, ctl, model, learned ? 1 : 0, plan);
 }
 
 // Device buffers of one training / gradient evaluation.

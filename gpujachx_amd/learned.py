@@ -183,41 +183,66 @@ def _make_input(torch, m, n, nplus, seed, kind):
     return ((U * s) @ V.T).cpu().numpy()
 
 
-def evaluate(path, seeds=(201,), sizes=None, max_sweeps=80):
-    """Plain vs learned form on fresh matrices of the training shapes."""
+def _run_dev(torch, alg, Gt, nplus, D):
+    """One hypJacL2_dev call on a fresh copy of the column-major device
+    matrix Gt (n x m row-major view); returns wall seconds."""
+    m, n = Gt.shape[1], Gt.shape[0]
+    W = Gt.clone()
+    sw = ctypes.c_uint(0)
+    gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    tm = (ctypes.c_double * 4)()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = g.lib().hypJacL2_dev(alg, m, n, nplus, W.data_ptr(), m, None, 0, D.data_ptr(), ctypes.byref(sw),
+                              ctypes.byref(gs), ctypes.byref(gb), tm,
+                              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    if rc:
+        raise RuntimeError("hypJacL2_dev returned %d" % rc)
+    return time.perf_counter() - t0
+
+
+def evaluate(path, seeds=(201,), sizes=None, thresholds=(None,), max_skips=(None,), max_sweeps=80):
+    """Plain vs learned form (device API, warm) on fresh matrices of the
+    training shapes, for each (threshold, max_skips) setting."""
     import torch
     out = []
     for name, m, n, nplus, _ in _configs():
         if sizes and n not in sizes:
             continue
         for seed in seeds:
-            G = _make_input(torch, m, n, nplus, seed, "hsvd" if nplus < n else "svd")
+            Gt = torch.from_numpy(_make_input(torch, m, n, nplus, seed,
+                                              "hsvd" if nplus < n else "svd").T.copy()).cuda()
+            D = torch.empty(n, dtype=torch.float64, device="cuda")
             alg = 11 if nplus < n else 9
             g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
             old = g.set_max_sweeps(max_sweeps)
             try:
                 g.set_form(g.FORM_PLAIN)
-                t0 = time.time()
-                rp = g.hypJacL2(alg, G, nplus)
-                tp = time.time() - t0
-                _, _, vp = g.last_skips()
-                enable(path)
-                t0 = time.time()
-                rl = g.hypJacL2(alg, G, nplus)
-                tl = time.time() - t0
-                sk, tot, vl = g.last_skips()
-                _, rb, _ = g.last_sweeps()
+                _run_dev(torch, alg, Gt, nplus, D)
+                tp = _run_dev(torch, alg, Gt, nplus, D)
+                sp = np.sort(torch.sqrt(torch.abs(D)).cpu().numpy())
+                swp, vp = len(g.last_sweeps()[0]), g.last_skips()[2]
+                for thr in thresholds:
+                    for ms in max_skips:
+                        enable(path, threshold=thr, max_skips=ms)
+                        tl = _run_dev(torch, alg, Gt, nplus, D)
+                        sk, tot, vl = g.last_skips()
+                        _, rb, _ = g.last_sweeps()
+                        sl = np.sort(torch.sqrt(torch.abs(D)).cpu().numpy())
+                        rec = {"config": name, "seed": seed, "threshold": thr, "max_skips": ms,
+                               "plain_s": tp, "learned_s": tl, "speedup": tp / tl,
+                               "plain_sweeps": swp, "learned_sweeps": len(rb),
+                               "plain_visits": vp, "learned_visits": vl, "skipped": tot,
+                               "skipped_per_sweep": sk, "rot_b_per_sweep": rb,
+                               "ends_unskipped_converged": bool(sk[-1] == 0 and rb[-1] == 0),
+                               "sigma_normwise_diff": float(np.abs(sp - sl).max() / sp.max())}
+                        print(json.dumps(rec), flush=True)
+                        out.append(rec)
+                        g.set_form(g.FORM_PLAIN)
             finally:
                 g.set_form(g.FORM_PLAIN)
                 g.set_max_sweeps(old)
-            a, b = np.sort(rp.sigma), np.sort(rl.sigma)
-            rec = {"config": name, "seed": seed, "plain_s": tp, "learned_s": tl,
-                   "plain_sweeps": rp.sweeps, "learned_sweeps": rl.sweeps,
-                   "plain_visits": vp, "learned_visits": vl, "skipped": tot,
-                   "ends_unskipped_converged": bool(sk[-1] == 0 and rb[-1] == 0),
-                   "sigma_normwise_diff": float(np.abs(a - b).max() / a.max())}
-            print(json.dumps(rec), flush=True)
-            out.append(rec)
     return out
 
 
@@ -231,12 +256,16 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.05)
     ap.add_argument("--max-skips", type=int, default=2)
     ap.add_argument("--max-n", type=int, default=8192, help="largest training order")
+    ap.add_argument("--sizes", type=int, nargs="*", help="eval: orders to evaluate")
+    ap.add_argument("--thresholds", type=float, nargs="*", help="eval: skip thresholds")
+    ap.add_argument("--skips", type=int, nargs="*", help="eval: max consecutive skips")
     args = ap.parse_args()
     os.environ.setdefault("HJ_QUIET", "1")
     import torch
     g.configureGPU(0)
     if args.cmd == "eval":
-        evaluate(args.model)
+        evaluate(args.model, sizes=args.sizes, thresholds=args.thresholds or (None,),
+                 max_skips=args.skips or (None,))
         return
     Xs, ys, meta = [], [], {"configs": []}
     for name, m, n, nplus, seed in _configs():

@@ -143,6 +143,19 @@ unsigned hj_visit_record_size(void);
  * max_sweeps sweeps (NULL switches it off): the training traces. */
 void hj_set_visit_trace(void *host_buf, unsigned max_sweeps);
 
+/* Split-precision Gram (HJ_FORM_SPLIT_GRAM): a pair's Gram is formed from
+ * three-way BF16 splits on BF16 MFMA with FP32 accumulation and FP64
+ * recombination while its last max |cos| exceeds tau (default 1e-3) and in
+ * the first sweep; the rotations, the Cholesky, the updates and the
+ * convergence test stay FP64, and a run ends only on a sweep that is all
+ * FP64 (and unskipped) without a big rotation. */
+void hj_set_split_tau(float tau);
+/* Split-Gram visits per sweep of the last call and their total. */
+void hj_last_splits(unsigned *split, unsigned long long *total);
+/* Self-test: the 32x32 Gram of columns 0..31 of a column-major nrow x 32
+ * host matrix from the FP64 (split = 0) or split (split = 1) Gram kernel. */
+int hj_dbg_gram(int split, unsigned nrow, const double *hG, double *hA);
+
 /* Skipped visits per sweep of the last call, their total, executed visits. */
 void hj_last_skips(unsigned *skip, unsigned long long *skipped,
                    unsigned long long *visits);
