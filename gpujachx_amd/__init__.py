@@ -33,6 +33,25 @@ FORM_SPLIT_GRAM = 2
 _lib = None
 
 
+class BatchDesc(ctypes.Structure):
+    """hj_batch_desc: one problem of a batch (device pointers)."""
+    _fields_ = [("routine", ctypes.c_uint), ("nrow", ctypes.c_uint), ("ncol", ctypes.c_uint),
+                ("nplus", ctypes.c_uint), ("ldG", ctypes.c_uint), ("ldV", ctypes.c_uint),
+                ("G", ctypes.c_void_p), ("V", ctypes.c_void_p), ("D", ctypes.c_void_p)]
+
+
+class BatchCfg(ctypes.Structure):
+    """hj_batch_cfg: launch shape and schedule of a batch."""
+    _fields_ = [("waves", ctypes.c_uint), ("wg_per_cu", ctypes.c_uint), ("strategy", ctypes.c_uint),
+                ("max_sweeps", ctypes.c_uint), ("longest_first", ctypes.c_uint)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+BATCH_STRAT = {"cycwor": 0, "BrentL": 1, "mmstep": 2}
+
+
 def lib():
     """Load libhypjac.so (once).  Raises if it has not been built."""
     global _lib
@@ -100,6 +119,19 @@ def lib():
     L.hj_last_splits.restype = None
     L.hj_dbg_gram.argtypes = [i, u, pd, pd]
     L.hj_dbg_gram.restype = i
+    pdesc, pcfg = ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchCfg)
+    L.hj_batch_default_cfg.argtypes = [pcfg]
+    L.hj_batch_default_cfg.restype = None
+    L.hj_batch_plan.argtypes = [pdesc, u, pcfg, ctypes.POINTER(i)]
+    L.hj_batch_plan.restype = vp
+    L.hj_batch_run.argtypes = [vp, vp]
+    L.hj_batch_run.restype = i
+    L.hj_batch_results.argtypes = [vp, pu, pull, pull, pd, pd]
+    L.hj_batch_results.restype = i
+    L.hj_batch_free.argtypes = [vp]
+    L.hj_batch_free.restype = None
+    L.hj_batch_autotune.argtypes = [pdesc, u, u, pcfg, pd]
+    L.hj_batch_autotune.restype = i
     _lib = L
     return L
 
@@ -268,3 +300,61 @@ def dbg_scalar(kind, a, b=None, c=None):
     if rc:
         raise RuntimeError("hj_dbg_scalar failed")
     return oc, ot, ob
+
+
+class Batch:
+    """A planned batch of independent (H)SVDs on device buffers (one
+    persistent launch captured in a hipGraph).  `descs` is a list of
+    BatchDesc; run() executes it on `stream` (a hipStream_t handle or None)."""
+
+    def __init__(self, descs, cfg=None):
+        L = lib()
+        self.n = len(descs)
+        self._arr = (BatchDesc * self.n)(*descs)
+        err = ctypes.c_int(0)
+        self.cfg = cfg if cfg is not None else default_batch_cfg()
+        self._plan = L.hj_batch_plan(self._arr, self.n, ctypes.byref(self.cfg), ctypes.byref(err))
+        if not self._plan:
+            raise ValueError("hj_batch_plan failed with %d" % err.value)
+
+    def run(self, stream=None):
+        if lib().hj_batch_run(self._plan, stream):
+            raise RuntimeError("hj_batch_run failed")
+
+    def results(self):
+        n = self.n
+        sw = (ctypes.c_uint * n)()
+        rs, rb = (ctypes.c_ulonglong * n)(), (ctypes.c_ulonglong * n)()
+        t0, t1 = np.zeros(n), np.zeros(n)
+        lib().hj_batch_results(self._plan, sw, rs, rb, _pd(t0), _pd(t1))
+        return {"sweeps": np.array(sw[:]), "rot_s": np.array(rs[:]), "rot_b": np.array(rb[:]),
+                "start_us": t0, "end_us": t1}
+
+    def close(self):
+        if self._plan:
+            lib().hj_batch_free(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_batch_cfg():
+    c = BatchCfg()
+    lib().hj_batch_default_cfg(ctypes.byref(c))
+    return c
+
+
+def batch_autotune(descs, sample=256, max_sweeps=None):
+    """(best BatchCfg, problems/s on the sample)."""
+    arr = (BatchDesc * len(descs))(*descs)
+    best = default_batch_cfg()
+    if max_sweeps:
+        best.max_sweeps = max_sweeps
+    rate = ctypes.c_double(0.0)
+    if lib().hj_batch_autotune(arr, len(descs), sample, ctypes.byref(best), ctypes.byref(rate)):
+        raise RuntimeError("hj_batch_autotune failed")
+    return best, rate.value

@@ -1,0 +1,754 @@
+// hj_batch.hip -- many independent (H)SVDs of uneven sizes in one persistent
+// launch.
+//
+// Every workgroup takes whole problems from an atomic queue (longest first)
+// and runs the blocked one-sided Jacobi method on each: level-2 sweeps in
+// strategy order, one level-1 step at a time; the block pairs of a step are
+// spread over the workgroup's waves, one wave doing everything for its pair
+// (FP64 MFMA Gram over all rows, 32x32 Cholesky, the level-0 inner sweeps on
+// R in the wave's LDS region, FP64 MFMA update of G and V), and a workgroup
+// barrier separates the steps.  Workgroups never wait on each other, so the
+// grid need not be resident.  The arithmetic of a problem is that of the
+// single-problem path with the Gram of a pair formed by one wave over all
+// rows in order (the oracle's single-wave Gram order), so each problem is
+// checked bit for bit against the oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hypjac.h"
+#include "hj_device.hpp"
+#include "hj_strat.hpp"
+#include "hj_util.hpp"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace hj {
+namespace {
+
+constexpr unsigned BLDR = 33u;                        // padded leading dimension
+constexpr unsigned WAVE_LDS = 2u * 32u * BLDR;        // doubles of R and W per wave
+constexpr unsigned BATCH_MAX_WAVES = 8u;
+
+struct BProb {
+  double *G, *V, *D;
+  unsigned nrow, ncol, nplus, ldG, ldV, routine;
+  unsigned s1off;  // first pair of this problem's strategy table in the pool
+  unsigned steps, np;
+  unsigned pad;
+};
+
+struct BOut {
+  unsigned sweeps, pad;
+  unsigned long long rot_s, rot_b;
+  unsigned long long t0, t1;  // wall clock (100 MHz) at start and end
+};
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+#define BR(A, r, c) (A)[(c) * BLDR + (r)]
+
+// Sum of the 32 rows of one pivot pair held by 4 lanes, lane r holding rows
+// r + 4k as v[k]: the xor butterfly 16,8,4,2,1 of the reference's 32-lane
+// reduction (levels 16, 8 and 4 inside the lane, 2 and 1 across lanes).
+__device__ __forceinline__ double sum4x8(const double *v)
+{
+  const double a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
+  double s = (a0 + a2) + (a1 + a3);
+  s = s + lane_xor2(s);
+  s = s + lane_xor1(s);
+  return s;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v += (unsigned)__shfl_xor((int)v, o, 64);
+  return v;
+}
+
+// Gram of the pair (columns 16 iBlk.., 16 jBlk..) over all nrow rows by one
+// wave: 64-row groups in order, MFMA steps (j, e) taking row 8j + 2k + e in
+// k-slot k (the order of the single-problem Gram with one chunk and one
+// wave); result into R (both 16x16 diagonal tiles and the lower off-diagonal
+// tile).
+__device__ void wave_gram(const double *__restrict__ G, unsigned ldG, unsigned nrow, unsigned iBlk,
+                          unsigned jBlk, double *R, unsigned l)
+{
+  const unsigned r0 = 2u * (l >> 4);
+  const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
+  const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
+  d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
+  for (unsigned g = 0u; g < nrow; g += 64u) {
+    double2 x0[8], x1[8];
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      x0[j] = *reinterpret_cast<const double2 *>(g0 + g + 8u * j);
+      x1[j] = *reinterpret_cast<const double2 *>(g1 + g + 8u * j);
+    }
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      a00 = mfma64(x0[j].x, x0[j].x, a00);
+      a10 = mfma64(x1[j].x, x0[j].x, a10);
+      a11 = mfma64(x1[j].x, x1[j].x, a11);
+      a00 = mfma64(x0[j].y, x0[j].y, a00);
+      a10 = mfma64(x1[j].y, x0[j].y, a10);
+      a11 = mfma64(x1[j].y, x1[j].y, a11);
+    }
+  }
+  // lane l, register r holds element (4r + l/16, l%16) of each tile
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    const unsigned i = 4u * r + (l >> 4), j = l & 15u;
+    BR(R, i, j) = a00[r];
+    BR(R, 16u + i, j) = a10[r];
+    BR(R, 16u + i, 16u + j) = a11[r];
+  }
+}
+
+// [M_i M_j] <- [M_i M_j] W over rows [0, nrow) by one wave, 16-row slabs
+// (the arithmetic of k_update: per element an fma chain over k = 0..31).
+__device__ void wave_update(double *__restrict__ M, unsigned ld, unsigned nrow, unsigned iBlk,
+                            unsigned jBlk, const double *W, unsigned l)
+{
+  double a0[8], a1[8];
+#pragma unroll
+  for (unsigned u = 0u; u < 8u; ++u) {
+    a0[u] = BR(W, 4u * u + (l >> 4), (l & 15u));
+    a1[u] = BR(W, 4u * u + (l >> 4), 16u + (l & 15u));
+  }
+  double *cp[8];
+#pragma unroll
+  for (unsigned u = 0u; u < 8u; ++u) {
+    const unsigned cc = 4u * u + (l >> 4);
+    const unsigned gc = (cc < 16u) ? (16u * iBlk + cc) : (16u * jBlk + cc - 16u);
+    cp[u] = M + (size_t)gc * ld + (l & 15u);
+  }
+  double *op0[4], *op1[4];
+#pragma unroll
+  for (unsigned q = 0u; q < 4u; ++q) {
+    const unsigned cc = 4u * q + (l >> 4);
+    op0[q] = M + (size_t)(16u * iBlk + cc) * ld + (l & 15u);
+    op1[q] = M + (size_t)(16u * jBlk + cc) * ld + (l & 15u);
+  }
+  double nb[8];
+#pragma unroll
+  for (unsigned u = 0u; u < 8u; ++u)
+    nb[u] = cp[u][0];
+  for (unsigned r = 0u; r < nrow; r += 16u) {
+    double b[8];
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u)
+      b[u] = nb[u];
+    if (r + 16u < nrow) {
+#pragma unroll
+      for (unsigned u = 0u; u < 8u; ++u)
+        nb[u] = cp[u][r + 16u];
+    }
+    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u) {
+      acc0 = mfma64(a0[u], b[u], acc0);
+      acc1 = mfma64(a1[u], b[u], acc1);
+    }
+#pragma unroll
+    for (unsigned q = 0u; q < 4u; ++q) {
+      op0[q][r] = acc0[q];
+      op1[q][r] = acc1[q];
+    }
+  }
+}
+
+// One block pair by one wave: Gram, Cholesky, level-0 sweep(s), update.
+// Returns the rotation counts of the pair (s, b) in every lane.
+template <bool SORT>
+__device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
+                          const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
+                          unsigned l, unsigned &out_s, unsigned &out_b)
+{
+  // local J structure, share/device_code_nosort_accumV.hpp:53-76
+  bool hyp = false;
+  int def = definite;
+  unsigned npos = 0u;
+  if (!definite) {
+    const unsigned ifc0 = iBlk * 16u, ifc1 = jBlk * 16u, nplus = P.nplus;
+    if (nplus <= ifc0) { npos = 0u; def = -1; }
+    else if (nplus <= ifc0 + 16u) { npos = nplus - ifc0; def = 0; }
+    else if (nplus <= ifc1) { npos = 16u; def = 0; }
+    else if (nplus <= ifc1 + 16u) { npos = 16u + (nplus - ifc1); def = 0; }
+    else { npos = 32u; def = +1; }
+    hyp = (def == 0);
+  }
+
+  wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
+#pragma unroll
+  for (unsigned e = 0u; e < 16u; ++e) {
+    const unsigned k = l + 64u * e;
+    BR(W, k & 31u, k >> 5) = ((k & 31u) == (k >> 5)) ? 1.0 : 0.0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // Cholesky, share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT)
+  {
+    const unsigned x = l & 31u;
+    double a[32];
+#pragma unroll
+    for (unsigned j = 0u; j < 32u; ++j)
+      a[j] = (j <= x) ? BR(R, x, j) : 0.0;
+#pragma unroll
+    for (unsigned k = 0u; k < 32u; ++k) {
+      const double akk = readlane_d(a[k], k);
+      const double rk = dev_rsqrt(akk);
+      const double dk = sqrt(akk);
+      if (x > k) a[k] = a[k] * rk;
+      else if (x == k) a[k] = dk;
+#pragma unroll
+      for (unsigned j = k + 1u; j < 32u; ++j) {
+        const double ljk = readlane_d(a[k], j);
+        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (l < 32u) {
+#pragma unroll
+      for (unsigned y = 0u; y < 32u; ++y)
+        BR(R, y, x) = (y <= x) ? a[y] : 0.0;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // level-0 sweeps: slot y = l/4 (16 pivot pairs), lane r = l%4 of the slot
+  // holds rows r + 4k of the pair's columns of R and W
+  const unsigned y = l >> 2, r = l & 3u;
+  unsigned blk_s = 0u, blk_b = 0u;
+  for (unsigned swp = 0u; swp < nswp; ++swp) {
+    unsigned my_s = 0u, my_b = 0u;
+    for (unsigned step = 0u; step < s0steps; ++step) {
+      const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
+      double Gp[8], Gq[8], Vp[8], Vq[8], t0[8], t1[8], t2[8];
+#pragma unroll
+      for (unsigned k = 0u; k < 8u; ++k) {
+        Gp[k] = BR(R, r + 4u * k, p);
+        Gq[k] = BR(R, r + 4u * k, q);
+        Vp[k] = BR(W, r + 4u * k, p);
+        Vq[k] = BR(W, r + 4u * k, q);
+      }
+#pragma unroll
+      for (unsigned k = 0u; k < 8u; ++k) {
+        t0[k] = __fma_rn(Gp[k], Gp[k], 0.0);
+        t1[k] = __fma_rn(Gq[k], Gq[k], 0.0);
+        t2[k] = __fma_rn(Gp[k], Gq[k], 0.0);
+      }
+      double Dp = sum4x8(t0), Dq = sum4x8(t1);
+      const double Apq = sum4x8(t2);
+      const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
+      const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
+      int pdef = def;
+      if (hyp) {
+        if ((p < npos) && (q < npos)) pdef = +1;
+        else if ((p >= npos) && (q >= npos)) pdef = -1;
+        else pdef = 0;
+      }
+      int transf_b = 0;
+      __builtin_amdgcn_wave_barrier();
+      if (transf_s) {
+        double c, t, t_;
+        bool trig;
+        if (!SORT) trig = !hyp || pdef;
+        else trig = (pdef != 0);
+        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
+        t_ = trig ? -t : t;
+        if (SORT && trig) {
+          Dp = __fma_rn(t, Apq, Dp);
+          Dq = __fma_rn(t_, Apq, Dq);
+        }
+        bool keep = true;
+        if (SORT)
+          keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
+        const unsigned PP = keep ? p : q, QQ = keep ? q : p;
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k) {
+          const double rp = __fma_rn(t_, Gq[k], Gp[k]), rq = __fma_rn(t, Gp[k], Gq[k]);
+          const double wp = __fma_rn(t_, Vq[k], Vp[k]), wq = __fma_rn(t, Vp[k], Vq[k]);
+          BR(R, r + 4u * k, PP) = transf_b ? c * rp : rp;
+          BR(R, r + 4u * k, QQ) = transf_b ? c * rq : rq;
+          BR(W, r + 4u * k, PP) = transf_b ? c * wp : wp;
+          BR(W, r + 4u * k, QQ) = transf_b ? c * wq : wq;
+        }
+      }
+      else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k) {
+          BR(R, r + 4u * k, p) = Gq[k];
+          BR(R, r + 4u * k, q) = Gp[k];
+          BR(W, r + 4u * k, p) = Vq[k];
+          BR(W, r + 4u * k, q) = Vp[k];
+        }
+      }
+      my_s += (unsigned)transf_s;
+      my_b += (unsigned)transf_b;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // every lane of a slot holds the slot's counts: sum over the wave / 4
+    const unsigned swp_s = wave_sum_u(my_s) >> 2, swp_b = wave_sum_u(my_b) >> 2;
+    if (swp_s) {
+      blk_s += swp_s;
+      blk_b += swp_b;
+    }
+    else
+      break;
+  }
+  out_s = blk_s;
+  out_b = blk_b;
+  if (SORT || blk_s) {
+    wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l);
+    if (P.routine & HYPJAC_FULL_SVD)
+      wave_update(P.V, P.ldV, P.ncol, iBlk, jBlk, W, l);
+  }
+}
+
+__global__ void k_batch_reset(unsigned *queue)
+{
+  if (threadIdx.x == 0u) *queue = 0u;
+}
+
+__global__ __launch_bounds__(512) void k_batch(const BProb *__restrict__ probs,
+                                               const unsigned *__restrict__ order, unsigned nprob,
+                                               unsigned *__restrict__ queue,
+                                               const uint16_t *__restrict__ s1pool,
+                                               const uint8_t *__restrict__ strat0, unsigned s0steps,
+                                               unsigned max_sweeps, BOut *__restrict__ out)
+{
+  extern __shared__ double lds[];
+  __shared__ uint8_t S0[1024];
+  __shared__ unsigned s_task;
+  __shared__ unsigned cnt[2u * BATCH_MAX_WAVES];
+  const unsigned l = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double *const R = lds + (size_t)wv * WAVE_LDS;
+  double *const W = R + 32u * BLDR;
+  for (unsigned e = threadIdx.x; e < s0steps * 32u; e += blockDim.x)
+    S0[e] = strat0[e];
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0u)
+      s_task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const unsigned task = s_task;
+    if (task >= nprob)
+      break;
+    const unsigned pid = order[task];
+    const BProb P = probs[pid];
+    unsigned long long t0 = 0ull;
+    if (threadIdx.x == 0u)
+      t0 = wall_clock64();
+    const bool sort = (P.routine & HYPJAC_CDSORT_1) != 0u;
+    const bool full = (P.routine & HYPJAC_FULL_SVD) != 0u;
+    const unsigned nswp = (P.routine & HYPJAC_BLK_ORI) ? 1u : HYPJAC_NSWEEP;
+    const int definite = P.nplus ? ((P.nplus < P.ncol) ? 0 : +1) : -1;
+    if (full) {
+      // V <- I (columns 0..ncol-1, rows 0..ncol-1)
+      const size_t tot = (size_t)P.ncol * P.ncol;
+      for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        const unsigned i = (unsigned)(e % P.ncol), j = (unsigned)(e / P.ncol);
+        P.V[(size_t)j * P.ldV + i] = (i == j) ? 1.0 : 0.0;
+      }
+    }
+    unsigned long long tot_s = 0ull, tot_b = 0ull;
+    unsigned sweeps = 0u;
+    const uint16_t *const tab = s1pool + 2u * (size_t)P.s1off;
+    while (sweeps < max_sweeps) {
+      unsigned sw_s = 0u, sw_b = 0u;
+      for (unsigned st = 0u; st < P.steps; ++st) {
+        __syncthreads();
+        for (unsigned pi = wv; pi < P.np; pi += nw) {
+          const unsigned iBlk = tab[2u * (st * P.np + pi)], jBlk = tab[2u * (st * P.np + pi) + 1u];
+          unsigned s = 0u, b = 0u;
+          if (sort)
+            wave_pair<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+          else
+            wave_pair<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+          sw_s += s;
+          sw_b += b;
+        }
+      }
+      if (l == 0u) {
+        cnt[2u * wv] = sw_s;
+        cnt[2u * wv + 1u] = sw_b;
+      }
+      __syncthreads();
+      unsigned cs = 0u, cb = 0u;
+      for (unsigned v = 0u; v < nw; ++v) {
+        cs += cnt[2u * v];
+        cb += cnt[2u * v + 1u];
+      }
+      ++sweeps;
+      tot_s += cs;
+      tot_b += cb;
+      __syncthreads();
+      if (!cb)
+        break;
+    }
+    // initD: one wave per column, the arithmetic of k_initD
+    for (unsigned cix = wv; cix < P.ncol; cix += nw) {
+      double *const g = P.G + (size_t)cix * P.ldG;
+      double x = 0.0;
+      for (unsigned r = l; r < P.nrow; r += 64u) {
+        const double v = g[r];
+        x = __fma_rn(v, v, x);
+      }
+      x = x + __shfl_xor(x, 32);
+      x = x + __shfl_xor(x, 16);
+      x = x + __shfl_xor(x, 8);
+      x = x + __shfl_xor(x, 4);
+      x = x + __shfl_xor(x, 2);
+      x = x + __shfl_xor(x, 1);
+      const double s = sqrt(x);
+      for (unsigned r = l; r < P.nrow; r += 64u)
+        g[r] = g[r] / s;
+      if (!l)
+        P.D[cix] = (cix < P.nplus) ? x : -x;
+    }
+    if (threadIdx.x == 0u) {
+      BOut o;
+      o.sweeps = sweeps;
+      o.pad = 0u;
+      o.rot_s = tot_s;
+      o.rot_b = tot_b;
+      o.t0 = t0;
+      o.t1 = wall_clock64();
+      out[pid] = o;
+    }
+  }
+}
+
+// strategy families the batch may use for an order n1 (built-in generators)
+bool family_for(unsigned strategy, unsigned n1, std::string &fam)
+{
+  const char *pref = (strategy == HJ_BATCH_STRAT_MMSTEP) ? "mmstep"
+                     : (strategy == HJ_BATCH_STRAT_BRENTL) ? "BrentL" : "cycwor";
+  if (can_generate(pref, n1)) { fam = pref; return true; }
+  if (can_generate("BrentL", n1)) { fam = "BrentL"; return true; }
+  if (can_generate("mmstep", n1)) { fam = "mmstep"; return true; }
+  return false;
+}
+
+struct Plan {
+  unsigned n = 0u;
+  hj_batch_cfg cfg;
+  BProb *d_probs = nullptr;
+  unsigned *d_order = nullptr, *d_queue = nullptr;
+  uint16_t *d_pool = nullptr;
+  uint8_t *d_s0 = nullptr;
+  unsigned s0steps = 0u;
+  BOut *d_out = nullptr;
+  std::vector<BOut> h_out;
+  std::vector<BProb> h_probs;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t cap = nullptr;
+  unsigned grid = 0u;
+  size_t lds = 0u;
+  double clock_mhz = 100.0;
+};
+
+double cost_of(const hj_batch_desc &d)
+{
+  const double nv = (d.routine & HYPJAC_FULL_SVD) ? d.ncol : 0.0;
+  return (double)d.ncol * d.ncol * (2.0 * d.nrow + nv);
+}
+
+}  // namespace
+}  // namespace hj
+
+using namespace hj;
+
+extern "C" void hj_batch_default_cfg(hj_batch_cfg *cfg)
+{
+  if (!cfg) return;
+  cfg->waves = 4u;
+  cfg->wg_per_cu = 2u;
+  cfg->strategy = HJ_BATCH_STRAT_CYCWOR;
+  cfg->max_sweeps = HYPJAC_NSWEEP;
+  cfg->longest_first = 1u;
+}
+
+static int check_desc(const hj_batch_desc &d)
+{
+  if (d.routine >= 16u) return -1;
+  if (!d.nrow || (d.nrow % 64u)) return -2;
+  if (!d.ncol || (d.ncol > d.nrow) || (d.ncol % 32u) || d.ncol > 16u * STRAT1_MAX_STEPS) return -3;
+  if (d.nplus > d.ncol) return -4;
+  if (!d.G) return -5;
+  if (d.ldG < d.nrow || (d.ldG & 1u) || (reinterpret_cast<uintptr_t>(d.G) & 15u)) return -6;
+  if (d.routine & HYPJAC_FULL_SVD) {
+    if (!d.V) return -7;
+    if (d.ldV < d.ncol || (d.ldV & 1u)) return -8;
+  }
+  if (!d.D) return -9;
+  return 0;
+}
+
+extern "C" void hj_batch_free(void *plan);
+
+extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_batch_cfg *cfg_in,
+                               int *err)
+{
+  int e = 0;
+  if (!probs || !n) e = -20;
+  hj_batch_cfg cfg;
+  hj_batch_default_cfg(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (!e && (cfg.waves < 1u || cfg.waves > BATCH_MAX_WAVES || !cfg.wg_per_cu || !cfg.max_sweeps))
+    e = -21;
+  for (unsigned i = 0u; !e && i < n; ++i)
+    e = check_desc(probs[i]);
+  if (e) {
+    if (err) *err = e;
+    return nullptr;
+  }
+  Plan *P = new Plan;
+  P->n = n;
+  P->cfg = cfg;
+  // strategy tables: one per distinct (family, n1)
+  std::vector<uint16_t> pool;
+  std::vector<std::pair<std::string, unsigned>> keys;
+  std::vector<unsigned> key_off, key_steps;
+  P->h_probs.resize(n);
+  for (unsigned i = 0u; i < n; ++i) {
+    const hj_batch_desc &d = probs[i];
+    const unsigned n1 = d.ncol / 16u;
+    std::string fam;
+    if (!family_for(cfg.strategy, n1, fam))
+      HJ_DIE("hj_batch_plan: no strategy of this order");
+    unsigned k = 0u;
+    while (k < keys.size() && !(keys[k].first == fam && keys[k].second == n1)) ++k;
+    if (k == keys.size()) {
+      Strategy t;
+      if (!generate(fam.c_str(), n1, t) || t.pairs != n1 / 2u)
+        HJ_DIE("hj_batch_plan: cannot generate a strategy table");
+      keys.push_back({fam, n1});
+      key_off.push_back((unsigned)(pool.size() / 2u));
+      key_steps.push_back(t.steps);
+      pool.insert(pool.end(), t.tab.begin(), t.tab.end());
+    }
+    BProb b;
+    b.G = d.G;
+    b.V = (d.routine & HYPJAC_FULL_SVD) ? d.V : nullptr;
+    b.D = d.D;
+    b.nrow = d.nrow;
+    b.ncol = d.ncol;
+    b.nplus = d.nplus;
+    b.ldG = d.ldG;
+    b.ldV = d.ldV;
+    b.routine = d.routine;
+    b.s1off = key_off[k];
+    b.steps = key_steps[k];
+    b.np = n1 / 2u;
+    b.pad = 0u;
+    P->h_probs[i] = b;
+  }
+  std::vector<unsigned> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  if (cfg.longest_first)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](unsigned a, unsigned b) { return cost_of(probs[a]) > cost_of(probs[b]); });
+  Strategy s0;
+  if (!generate("cycwor", 32u, s0) || s0.pairs != 16u || s0.steps > 32u)
+    HJ_DIE("hj_batch_plan: cannot generate cycwor 32");
+  std::vector<uint8_t> s0b(1024u, 0u);
+  for (size_t i = 0u; i < s0.tab.size() && i < 1024u; ++i) s0b[i] = (uint8_t)s0.tab[i];
+  P->s0steps = s0.steps;
+
+  HIP_CALL(hipMalloc(&P->d_probs, sizeof(BProb) * n));
+  HIP_CALL(hipMalloc(&P->d_order, sizeof(unsigned) * n));
+  HIP_CALL(hipMalloc(&P->d_queue, sizeof(unsigned)));
+  HIP_CALL(hipMalloc(&P->d_pool, sizeof(uint16_t) * std::max<size_t>(pool.size(), 2u)));
+  HIP_CALL(hipMalloc(&P->d_s0, 1024u));
+  HIP_CALL(hipMalloc(&P->d_out, sizeof(BOut) * n));
+  HIP_CALL(hipMemcpy(P->d_probs, P->h_probs.data(), sizeof(BProb) * n, hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(P->d_order, order.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(P->d_pool, pool.data(), sizeof(uint16_t) * pool.size(), hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(P->d_s0, s0b.data(), 1024u, hipMemcpyHostToDevice));
+  P->h_out.resize(n);
+
+  int dev = 0;
+  HIP_CALL(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_CALL(hipGetDeviceProperties(&prop, dev));
+  P->grid = cfg.wg_per_cu * (unsigned)prop.multiProcessorCount;
+  P->lds = sizeof(double) * WAVE_LDS * cfg.waves;
+  const size_t lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+  if (P->lds + 1100u > lds_max) {
+    // LDS of a workgroup beyond the device limit (workgroups beyond the
+    // per-CU capacity are not an error: they wait for a free slot)
+    hj_batch_free(P);
+    if (err) *err = -22;
+    return nullptr;
+  }
+  HIP_CALL(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds));
+  int rate_khz = 0;
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && rate_khz > 0)
+    P->clock_mhz = rate_khz / 1000.0;
+
+  // the batch as one graph: queue reset (a kernel, no memset node) and the
+  // persistent launch
+  HIP_CALL(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+  hipGraph_t graph;
+  HIP_CALL(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, P->cap, P->d_queue);
+  hipLaunchKernelGGL(k_batch, dim3(P->grid), dim3(64u * cfg.waves), P->lds, P->cap, P->d_probs,
+                     P->d_order, n, P->d_queue, P->d_pool, P->d_s0, P->s0steps, cfg.max_sweeps,
+                     P->d_out);
+  HIP_CALL(hipStreamEndCapture(P->cap, &graph));
+  HIP_CALL(hipGraphInstantiate(&P->gexec, graph, nullptr, nullptr, 0));
+  HIP_CALL(hipGraphDestroy(graph));
+  if (err) *err = 0;
+  return P;
+}
+
+extern "C" int hj_batch_run(void *plan, void *stream)
+{
+  Plan *P = static_cast<Plan *>(plan);
+  if (!P) return -1;
+  hipStream_t caller = static_cast<hipStream_t>(stream);
+  hipEvent_t ev;
+  HIP_CALL(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIP_CALL(hipEventRecord(ev, caller));
+  HIP_CALL(hipStreamWaitEvent(P->cap, ev, 0));
+  HIP_CALL(hipGraphLaunch(P->gexec, P->cap));
+  HIP_CALL(hipEventRecord(ev, P->cap));
+  HIP_CALL(hipStreamWaitEvent(caller, ev, 0));
+  HIP_CALL(hipStreamSynchronize(P->cap));
+  HIP_CALL(hipEventDestroy(ev));
+  HIP_CALL(hipMemcpy(P->h_out.data(), P->d_out, sizeof(BOut) * P->n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int hj_batch_results(void *plan, unsigned *sweeps, unsigned long long *rot_s,
+                                unsigned long long *rot_b, double *start_us, double *end_us)
+{
+  Plan *P = static_cast<Plan *>(plan);
+  if (!P) return -1;
+  unsigned long long tmin = ~0ull;
+  for (const BOut &o : P->h_out) tmin = std::min(tmin, o.t0);
+  for (unsigned i = 0u; i < P->n; ++i) {
+    const BOut &o = P->h_out[i];
+    if (sweeps) sweeps[i] = o.sweeps;
+    if (rot_s) rot_s[i] = o.rot_s;
+    if (rot_b) rot_b[i] = o.rot_b;
+    if (start_us) start_us[i] = (double)(o.t0 - tmin) / P->clock_mhz;
+    if (end_us) end_us[i] = (double)(o.t1 - tmin) / P->clock_mhz;
+  }
+  return 0;
+}
+
+extern "C" void hj_batch_free(void *plan)
+{
+  Plan *P = static_cast<Plan *>(plan);
+  if (!P) return;
+  if (P->gexec) HIP_CALL(hipGraphExecDestroy(P->gexec));
+  if (P->cap) HIP_CALL(hipStreamDestroy(P->cap));
+  for (void *p : {(void *)P->d_probs, (void *)P->d_order, (void *)P->d_queue, (void *)P->d_pool,
+                  (void *)P->d_s0, (void *)P->d_out})
+    if (p) HIP_CALL(hipFree(p));
+  delete P;
+}
+
+// Autotuner: times every candidate configuration on a size-stratified sample
+// of the batch (copies of its matrices, so the caller's data are untouched)
+// and returns the fastest.
+extern "C" int hj_batch_autotune(const hj_batch_desc *probs, unsigned n, unsigned sample,
+                                 hj_batch_cfg *best, double *best_rate)
+{
+  if (!probs || !n || !best) return -1;
+  for (unsigned i = 0u; i < n; ++i)
+    if (check_desc(probs[i])) return check_desc(probs[i]);
+  if (!sample || sample > n) sample = n;
+  std::vector<unsigned> idx(n);
+  std::iota(idx.begin(), idx.end(), 0u);
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](unsigned a, unsigned b) { return cost_of(probs[a]) > cost_of(probs[b]); });
+  std::vector<hj_batch_desc> sub;
+  const double stride = (double)n / sample;
+  for (unsigned k = 0u; k < sample; ++k) sub.push_back(probs[idx[(unsigned)(k * stride)]]);
+  // private copies: G (and V, D) of the sample, plus pristine G
+  std::vector<double *> pristine(sub.size());
+  std::vector<hj_batch_desc> work = sub;
+  for (size_t k = 0u; k < sub.size(); ++k) {
+    const size_t gb = sizeof(double) * (size_t)sub[k].ldG * sub[k].ncol;
+    HIP_CALL(hipMalloc(&pristine[k], gb));
+    HIP_CALL(hipMemcpy(pristine[k], sub[k].G, gb, hipMemcpyDeviceToDevice));
+    HIP_CALL(hipMalloc(&work[k].G, gb));
+    HIP_CALL(hipMalloc(&work[k].D, sizeof(double) * sub[k].ncol));
+    if (sub[k].routine & HYPJAC_FULL_SVD)
+      HIP_CALL(hipMalloc(&work[k].V, sizeof(double) * (size_t)sub[k].ldV * sub[k].ncol));
+  }
+  hj_batch_cfg base;
+  hj_batch_default_cfg(&base);
+  base.max_sweeps = best->max_sweeps ? best->max_sweeps : base.max_sweeps;
+  const unsigned waves[] = {2u, 4u, 8u};
+  const unsigned wgs[] = {1u, 2u, 4u};
+  const unsigned strats[] = {HJ_BATCH_STRAT_CYCWOR, HJ_BATCH_STRAT_BRENTL, HJ_BATCH_STRAT_MMSTEP};
+  double best_t = 1e300;
+  hj_batch_cfg bc = base;
+  hipEvent_t e0, e1;
+  HIP_CALL(hipEventCreate(&e0));
+  HIP_CALL(hipEventCreate(&e1));
+  hipStream_t s;
+  HIP_CALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  for (unsigned st : strats)
+    for (unsigned wv : waves)
+      for (unsigned wg : wgs) {
+        if (wv * wg > 8u) continue;  // LDS: at most 8 wave regions per CU
+        hj_batch_cfg c = base;
+        c.waves = wv;
+        c.wg_per_cu = wg;
+        c.strategy = st;
+        int err = 0;
+        void *pl = hj_batch_plan(work.data(), (unsigned)work.size(), &c, &err);
+        if (!pl) continue;
+        for (size_t k = 0u; k < sub.size(); ++k)
+          HIP_CALL(hipMemcpyAsync(work[k].G, pristine[k],
+                                  sizeof(double) * (size_t)sub[k].ldG * sub[k].ncol,
+                                  hipMemcpyDeviceToDevice, s));
+        HIP_CALL(hipEventRecord(e0, s));
+        hj_batch_run(pl, s);
+        HIP_CALL(hipEventRecord(e1, s));
+        HIP_CALL(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CALL(hipEventElapsedTime(&ms, e0, e1));
+        hj_batch_free(pl);
+        if (ms < best_t) {
+          best_t = ms;
+          bc = c;
+        }
+      }
+  HIP_CALL(hipStreamDestroy(s));
+  HIP_CALL(hipEventDestroy(e0));
+  HIP_CALL(hipEventDestroy(e1));
+  for (size_t k = 0u; k < sub.size(); ++k) {
+    HIP_CALL(hipFree(pristine[k]));
+    HIP_CALL(hipFree(work[k].G));
+    HIP_CALL(hipFree(work[k].D));
+    if (work[k].V && (sub[k].routine & HYPJAC_FULL_SVD)) HIP_CALL(hipFree(work[k].V));
+  }
+  *best = bc;
+  if (best_rate) *best_rate = sub.size() / (best_t * 1e-3);
+  return 0;
+}

@@ -19,7 +19,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 // Output rows of the 16x16 MFMA results (lane l, register r; the column is
 // l % 16), measured on gfx950: FP64 16x16x4 holds row 4r + l/16; FP32
-// 16x16x32 BF16 holds row 4(l/16) + r.
+// 16x16x32 BF16 holds row 4(l/16) + r (and its operands take k = 8(l/16) + t
+// from slot t of lane l).
 #define F32_OUT_ROW(l, r) (4u * ((l) >> 4) + (r))
 
 namespace hj {
@@ -39,7 +40,7 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 // MFMA steps (j, e) take row 8j+2k+e in k-slot k.  The next group's loads are
 // in flight while this group's MFMAs run.  Tiles: 00 = block i with itself,
 // 10 = block j with block i, 11 = block j with itself.
-__global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsigned ldG,
+__global__ __launch_bounds__(256, 2) void k_gram(const double *__restrict__ G, unsigned ldG,
                                               const uint16_t *__restrict__ pairs, unsigned nchunk,
                                               unsigned CH, double *__restrict__ part,
                                               const uint8_t *__restrict__ skip)
@@ -55,28 +56,31 @@ __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsi
   const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
   const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
-  double2 n0[8], n1[8];
+  // 32-row half groups (MFMA steps j = 4h .. 4h+3 of a 64-row group), the
+  // next half group in flight: half the registers of whole-group buffers,
+  // so twice the waves per SIMD to keep the FP64 MFMA pipe busy
+  double2 n0[4], n1[4];
 #pragma unroll
-  for (unsigned j = 0u; j < 8u; ++j) {
+  for (unsigned j = 0u; j < 4u; ++j) {
     n0[j] = *reinterpret_cast<const double2 *>(g0 + 8u * j);
     n1[j] = *reinterpret_cast<const double2 *>(g1 + 8u * j);
   }
-  for (unsigned g = 0u; g < SUB; g += 64u) {
-    double2 x0[8], x1[8];
+  for (unsigned g = 0u; g < SUB; g += 32u) {
+    double2 x0[4], x1[4];
 #pragma unroll
-    for (unsigned j = 0u; j < 8u; ++j) {
+    for (unsigned j = 0u; j < 4u; ++j) {
       x0[j] = n0[j];
       x1[j] = n1[j];
     }
-    if (g + 64u < SUB) {
+    if (g + 32u < SUB) {
 #pragma unroll
-      for (unsigned j = 0u; j < 8u; ++j) {
-        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 64u + 8u * j);
-        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 64u + 8u * j);
+      for (unsigned j = 0u; j < 4u; ++j) {
+        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 32u + 8u * j);
+        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 32u + 8u * j);
       }
     }
 #pragma unroll
-    for (unsigned j = 0u; j < 8u; ++j) {
+    for (unsigned j = 0u; j < 4u; ++j) {
       a00 = mfma(x0[j].x, x0[j].x, a00);
       a10 = mfma(x1[j].x, x0[j].x, a10);
       a11 = mfma(x1[j].x, x1[j].x, a11);
@@ -111,9 +115,10 @@ __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsi
 // Split-precision Gram partials (same grid, chunks, waves and output layout
 // as k_gram) for the pairs whose plan byte is PLAN_SPLIT.
 //
-// Per 64-row group a wave scales each block by a power of two (2^-e with
-// 2^(e-1) <= max |x| < 2^e over the group, so the scaled values lie in
-// (-1, 1) whatever the FP64 range), rounds to FP32 and splits every value
+// Per 32-row group a wave scales each column by a power of two (2^-e with
+// 2^(e-1) <= max |x| < 2^e over the column's 32 rows, so the scaled values
+// lie in (-1, 1) whatever the FP64 range and however the column scales of a
+// block differ), rounds to FP32 and splits every value
 // into three bfloat16 pieces x = h + m + l (h = bf16(x), m = bf16(x - h),
 // l = bf16(x - h - m): the FP32 value exactly, up to the last piece's
 // rounding).  The products of a tile are summed on BF16 MFMA 16x16x32 with
@@ -121,11 +126,11 @@ __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ G, unsi
 // (the dropped terms ml, lm, ll are below 2^-24 relative), and after each
 // group the classes are added smallest first in FP64, unscaled exactly by
 // 2^(e_a + e_b) and added to the FP64 accumulators.  Relative error of an
-// entry, against sqrt(A_ii A_jj): about 2^-22 per group, FP64 across groups.
+// entry, against sqrt(A_ii A_jj): about 2^-23 per group, FP64 across groups.
 //
-// Rows: lane l (column l%16, lane group q = l/16) holds rows 16q .. 16q+15 of
-// the group, 128 contiguous bytes per column; K-block 0 takes the first 8
-// (MFMA k = 8q + t), K-block 1 the last 8, so every row is taken once.
+// Rows: lane l (column l%16, lane group q = l/16) holds rows 8q .. 8q+7 of
+// the 32-row group (MFMA k = 8q + t), 64 contiguous bytes per lane and 256
+// per column; one MFMA per class and tile per group.
 typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
 typedef __attribute__((ext_vector_type(4))) float f4;
 
@@ -146,11 +151,11 @@ __device__ __forceinline__ int exp_field(double x)
   return (__double2hiint(x) >> 20) & 0x7ff;
 }
 
-__device__ __forceinline__ int wave_max_int(int v)
+// max over the four lanes l, l^16, l^32, l^48 (the lanes of one column)
+__device__ __forceinline__ int column_max_int(int v)
 {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1)
-    v = max(v, __shfl_xor(v, o, 64));
+  v = max(v, __shfl_xor(v, 16, 64));
+  v = max(v, __shfl_xor(v, 32, 64));
   return v;
 }
 
@@ -185,24 +190,26 @@ __device__ __forceinline__ void tile_products(Cls &C, const bf8 &ha, const bf8 &
   C.c2 = mfma_bf(ma, mb, C.c2);
 }
 
-__device__ __forceinline__ void fold(d4 &acc, Cls &C, int e)
+// adds the classes of a tile to its FP64 accumulator, element r (row
+// 4(l/16) + r, column l%16) unscaled by 2^(erow[r] + ecol)
+__device__ __forceinline__ void fold(d4 &acc, Cls &C, const int *erow, int ecol)
 {
 #pragma unroll
   for (unsigned r = 0u; r < 4u; ++r) {
     double t = (double)C.c2[r] + (double)C.c1[r];
     t = t + (double)C.c0[r];
-    acc[r] = acc[r] + ldexp(t, e);
+    acc[r] = acc[r] + ldexp(t, erow[r] + ecol);
     C.c0[r] = 0.0f;
     C.c1[r] = 0.0f;
     C.c2[r] = 0.0f;
   }
 }
 
-__global__ __launch_bounds__(256) void k_gram_split(const double *__restrict__ G, unsigned ldG,
-                                                    const uint16_t *__restrict__ pairs,
-                                                    unsigned nchunk, unsigned CH,
-                                                    double *__restrict__ part,
-                                                    const uint8_t *__restrict__ plan)
+__global__ __launch_bounds__(256, 2) void k_gram_split(const double *__restrict__ G, unsigned ldG,
+                                                       const uint16_t *__restrict__ pairs,
+                                                       unsigned nchunk, unsigned CH,
+                                                       double *__restrict__ part,
+                                                       const uint8_t *__restrict__ plan)
 {
   __shared__ d4 red[3][GRAM_WAVES - 1][64];
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -211,7 +218,7 @@ __global__ __launch_bounds__(256) void k_gram_split(const double *__restrict__ G
     return;
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
   const unsigned SUB = CH / nw;
-  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 16u * (l >> 4);
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 8u * (l >> 4);
   const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
   const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
@@ -219,50 +226,54 @@ __global__ __launch_bounds__(256) void k_gram_split(const double *__restrict__ G
   C00.c0 = C00.c1 = C00.c2 = f4{0.0f, 0.0f, 0.0f, 0.0f};
   C10 = C00;
   C11 = C00;
-  double2 n0[8], n1[8];
+  double2 n0[4], n1[4];
 #pragma unroll
-  for (unsigned j = 0u; j < 8u; ++j) {
+  for (unsigned j = 0u; j < 4u; ++j) {
     n0[j] = *reinterpret_cast<const double2 *>(g0 + 2u * j);
     n1[j] = *reinterpret_cast<const double2 *>(g1 + 2u * j);
   }
-  for (unsigned g = 0u; g < SUB; g += 64u) {
-    double x0[16], x1[16];
+  for (unsigned g = 0u; g < SUB; g += 32u) {
+    double x0[8], x1[8];
 #pragma unroll
-    for (unsigned j = 0u; j < 8u; ++j) {
+    for (unsigned j = 0u; j < 4u; ++j) {
       x0[2u * j] = n0[j].x;
       x0[2u * j + 1u] = n0[j].y;
       x1[2u * j] = n1[j].x;
       x1[2u * j + 1u] = n1[j].y;
     }
-    if (g + 64u < SUB) {
+    if (g + 32u < SUB) {
 #pragma unroll
-      for (unsigned j = 0u; j < 8u; ++j) {
-        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 64u + 2u * j);
-        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 64u + 2u * j);
+      for (unsigned j = 0u; j < 4u; ++j) {
+        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 32u + 2u * j);
+        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 32u + 2u * j);
       }
     }
     int e0 = 0, e1 = 0;
 #pragma unroll
-    for (unsigned t = 0u; t < 16u; ++t) {
+    for (unsigned t = 0u; t < 8u; ++t) {
       e0 = max(e0, exp_field(x0[t]));
       e1 = max(e1, exp_field(x1[t]));
     }
-    e0 = wave_max_int(e0);
-    e1 = wave_max_int(e1);
+    e0 = column_max_int(e0);
+    e1 = column_max_int(e1);
     e0 = e0 ? (e0 - 1022) : 0;
     e1 = e1 ? (e1 - 1022) : 0;
+    // exponents of the columns behind the rows this lane's results hold
+    int r0e[4], r1e[4];
 #pragma unroll
-    for (unsigned kb = 0u; kb < 2u; ++kb) {
-      bf8 h0, m0, q0, h1, m1, q1;
-      split3(x0 + 8u * kb, e0, h0, m0, q0);
-      split3(x1 + 8u * kb, e1, h1, m1, q1);
-      tile_products(C00, h0, m0, q0, h0, m0, q0);
-      tile_products(C10, h1, m1, q1, h0, m0, q0);
-      tile_products(C11, h1, m1, q1, h1, m1, q1);
+    for (unsigned r = 0u; r < 4u; ++r) {
+      r0e[r] = __shfl(e0, (int)f32_out_row(l, r), 64);
+      r1e[r] = __shfl(e1, (int)f32_out_row(l, r), 64);
     }
-    fold(a00, C00, e0 + e0);
-    fold(a10, C10, e1 + e0);
-    fold(a11, C11, e1 + e1);
+    bf8 h0, m0, q0, h1, m1, q1;
+    split3(x0, e0, h0, m0, q0);
+    split3(x1, e1, h1, m1, q1);
+    tile_products(C00, h0, m0, q0, h0, m0, q0);
+    tile_products(C10, h1, m1, q1, h0, m0, q0);
+    tile_products(C11, h1, m1, q1, h1, m1, q1);
+    fold(a00, C00, r0e, e0);
+    fold(a10, C10, r1e, e0);
+    fold(a11, C11, r1e, e1);
   }
   if (w) {
     red[0][w - 1u][l] = a00;

@@ -17,6 +17,8 @@ def main():
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--alg", type=int, default=9)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--form", choices=["plain", "split"], default="plain",
+                    help="split: every visit of the first sweep uses the split Gram")
     args = ap.parse_args()
     os.environ.setdefault("HJ_QUIET", "1")
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -36,14 +38,16 @@ def main():
     D = torch.empty(n, dtype=torch.float64, device="cuda")
     sw = ctypes.c_uint(0)
     gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    g.set_form(g.FORM_SPLIT_GRAM if args.form == "split" else g.FORM_PLAIN)
     g.set_step_limit(args.steps)
     rc = g.lib().hypJacL2_dev(args.alg, n, n, n, G0.data_ptr(), n, None, 0, D.data_ptr(),
                               ctypes.byref(sw), ctypes.byref(gs), ctypes.byref(gb), None,
                               torch.cuda.current_stream().cuda_stream)
     g.set_step_limit(0)
+    g.set_form(g.FORM_PLAIN)
     torch.cuda.synchronize()
-    print("profile_steps: n=%d steps=%d rc=%d copy=%d MiB" % (n, args.steps, rc, args.copy_mib),
-          flush=True)
+    print("profile_steps: n=%d steps=%d form=%s rc=%d copy=%d MiB"
+          % (n, args.steps, args.form, rc, args.copy_mib), flush=True)
 
 
 if __name__ == "__main__":

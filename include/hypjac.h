@@ -156,6 +156,48 @@ void hj_last_splits(unsigned *split, unsigned long long *total);
  * host matrix from the FP64 (split = 0) or split (split = 1) Gram kernel. */
 int hj_dbg_gram(int split, unsigned nrow, const double *hG, double *hA);
 
+/* ---- Many independent (H)SVDs in one persistent launch ---------------- */
+
+/* One problem: the arguments of hypJacL2_dev (device pointers; G and V are
+ * overwritten by U and V, D receives j_i sigma_i^2). */
+typedef struct hj_batch_desc {
+  unsigned routine, nrow, ncol, nplus, ldG, ldV;
+  double *G, *V, *D;
+} hj_batch_desc;
+
+#define HJ_BATCH_STRAT_CYCWOR 0u  /* cycwor where it exists, else BrentL */
+#define HJ_BATCH_STRAT_BRENTL 1u
+#define HJ_BATCH_STRAT_MMSTEP 2u
+
+/* Launch shape and schedule (hj_batch_autotune picks them). */
+typedef struct hj_batch_cfg {
+  unsigned waves;          /* waves per workgroup, 1..8 (one block pair each) */
+  unsigned wg_per_cu;      /* workgroups per CU in the persistent grid */
+  unsigned strategy;       /* HJ_BATCH_STRAT_* for the level-1 tables */
+  unsigned max_sweeps;     /* sweep cap per problem (reference: 30) */
+  unsigned longest_first;  /* queue order: largest M*N^2 first (else input order) */
+} hj_batch_cfg;
+
+void hj_batch_default_cfg(hj_batch_cfg *cfg);
+/* Builds the plan of a batch: strategy tables, problem descriptors and the
+ * batch captured as one hipGraph (queue reset + the persistent kernel).
+ * Returns NULL and *err < 0 on an invalid problem (the hypJacL2 codes of
+ * the first bad problem) or configuration. */
+void *hj_batch_plan(const hj_batch_desc *probs, unsigned n,
+                    const hj_batch_cfg *cfg, int *err);
+/* Runs the plan (ordered after `stream`'s prior work) and waits for it. */
+int hj_batch_run(void *plan, void *stream);
+/* Per problem: sweeps, rotation totals, device start/end time in us since
+ * the first problem started (NULL outputs are skipped). */
+int hj_batch_results(void *plan, unsigned *sweeps, unsigned long long *rot_s,
+                     unsigned long long *rot_b, double *start_us,
+                     double *end_us);
+void hj_batch_free(void *plan);
+/* Times every candidate launch shape and strategy on `sample` problems of
+ * the batch (copies; the inputs are untouched); best->max_sweeps is kept. */
+int hj_batch_autotune(const hj_batch_desc *probs, unsigned n, unsigned sample,
+                      hj_batch_cfg *best, double *best_rate);
+
 /* Skipped visits per sweep of the last call, their total, executed visits. */
 void hj_last_skips(unsigned *skip, unsigned long long *skipped,
                    unsigned long long *visits);

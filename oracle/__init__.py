@@ -52,6 +52,8 @@ def lib():
         L.hjo_hypJacL2.restype = i
         L.hjo_set_max_sweeps.argtypes = [u]
         L.hjo_set_max_sweeps.restype = u
+        L.hjo_set_single_wave_gram.argtypes = [i]
+        L.hjo_set_single_wave_gram.restype = i
         _L = L
     return _L
 
@@ -119,6 +121,11 @@ def hypJacL2(routine, G, nplus, strat0, strat1, ldhG=None, nthreads=0, max_steps
     return dict(ret=ret, G=hG[:, :nrow].T.copy(), V=None if hV is None else hV.T.copy(), D=hD,
                 sweeps=k, rot_s=gs.value, rot_b=gb.value,
                 rot_s_trace=list(ts[:k]), rot_b_trace=list(tb[:k]))
+
+
+def set_single_wave_gram(on):
+    """Gram order of the batched launch (one wave over all rows)."""
+    return lib().hjo_set_single_wave_gram(1 if on else 0)
 
 
 def set_max_sweeps(n):

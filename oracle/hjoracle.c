@@ -458,6 +458,15 @@ static void local_npos(unsigned nplus, unsigned iBlk, unsigned jBlk,
 }
 
 static unsigned g_max_sweeps = HJO_NSWEEP;
+static int g_single_wave_gram = 0;
+
+/* Gram order of the batched launch: one chunk, one wave over all rows. */
+int hjo_set_single_wave_gram(int on)
+{
+  const int old = g_single_wave_gram;
+  g_single_wave_gram = on ? 1 : 0;
+  return old;
+}
 unsigned hjo_set_max_sweeps(unsigned n)
 {
   const unsigned old = g_max_sweeps;
@@ -505,7 +514,7 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
 
   const unsigned nswp = blk_ori ? 1u : HJO_NSWEEP;
   const unsigned np = ncol / 32u;
-  const unsigned nchunk = hjo_gram_nchunk(nrow, np);
+  const unsigned nchunk = g_single_wave_gram ? 1u : hjo_gram_nchunk(nrow, np);
   const unsigned CH = nrow / nchunk;
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
 
@@ -542,8 +551,11 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
         double *const G0 = G + (size_t)iBlk * 16u * ldG;
         double *const G1 = G + (size_t)jBlk * 16u * ldG;
         double *const pp = part + (size_t)pi * 768u * nchunk;
-        for (unsigned c = 0u; c < nchunk; ++c)
-          hjo_gram_partial(G0, G1, ldG, c * CH, CH, pp + (size_t)c * 768u);
+        if (g_single_wave_gram)
+          gram_wave(G0, G1, ldG, 0u, nrow, pp);
+        else
+          for (unsigned c = 0u; c < nchunk; ++c)
+            hjo_gram_partial(G0, G1, ldG, c * CH, CH, pp + (size_t)c * 768u);
         int hyp = 0, def = definite;
         unsigned npos = 0u;
         if (!definite) {

@@ -119,6 +119,10 @@ unsigned hjo_set_max_sweeps(unsigned n);
  * buf == NULL switches tracing off. */
 void hjo_set_trace(unsigned *buf, size_t cap_entries);
 
+/* Gram order of the batched launch (gpujachx_amd/csrc/hj_batch.hip): the
+ * pair's Gram by one wave over all rows (no chunks).  Returns the old mode. */
+int hjo_set_single_wave_gram(int on);
+
 #ifdef __cplusplus
 }
 #endif
