@@ -78,32 +78,28 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v)
 }
 
 // Gram of the pair (columns 16 iBlk.., 16 jBlk..) over all nrow rows by one
-// wave: 64-row groups in order, MFMA steps (j, e) taking row 8j + 2k + e in
-// k-slot k (the order of the single-problem Gram with one chunk and one
-// wave); result into R (both 16x16 diagonal tiles and the lower off-diagonal
-// tile).
+// wave: 16-row slabs in order, MFMA step j taking row 4j + k in k-slot k
+// (the order of the single-problem Gram with one chunk and one wave); result
+// into R (both 16x16 diagonal tiles and the lower off-diagonal tile).
 __device__ void wave_gram(const double *__restrict__ G, unsigned ldG, unsigned nrow, unsigned iBlk,
                           unsigned jBlk, double *R, unsigned l)
 {
-  const unsigned r0 = 2u * (l >> 4);
+  const unsigned r0 = l >> 4;
   const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
   const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
-  for (unsigned g = 0u; g < nrow; g += 64u) {
-    double2 x0[8], x1[8];
+  for (unsigned g = 0u; g < nrow; g += 32u) {
+    double x0[8], x1[8];
 #pragma unroll
-    for (unsigned j = 0u; j < 8u; ++j) {
-      x0[j] = *reinterpret_cast<const double2 *>(g0 + g + 8u * j);
-      x1[j] = *reinterpret_cast<const double2 *>(g1 + g + 8u * j);
+    for (unsigned t = 0u; t < 8u; ++t) {
+      x0[t] = g0[g + 4u * t];
+      x1[t] = g1[g + 4u * t];
     }
 #pragma unroll
-    for (unsigned j = 0u; j < 8u; ++j) {
-      a00 = mfma64(x0[j].x, x0[j].x, a00);
-      a10 = mfma64(x1[j].x, x0[j].x, a10);
-      a11 = mfma64(x1[j].x, x1[j].x, a11);
-      a00 = mfma64(x0[j].y, x0[j].y, a00);
-      a10 = mfma64(x1[j].y, x0[j].y, a10);
-      a11 = mfma64(x1[j].y, x1[j].y, a11);
+    for (unsigned t = 0u; t < 8u; ++t) {
+      a00 = mfma64(x0[t], x0[t], a00);
+      a10 = mfma64(x1[t], x0[t], a10);
+      a11 = mfma64(x1[t], x1[t], a11);
     }
   }
   // lane l, register r holds element (4r + l/16, l%16) of each tile

@@ -84,6 +84,8 @@ struct Work {
   VisitRec *d_rec = nullptr;   // [steps][np] last executed visit of each pair
   uint8_t *d_skip = nullptr;   // [steps][np] skip mask of the running sweep
   uint16_t *d_model = nullptr; // bf16 MLP parameters
+  Quad *d_quads = nullptr;     // [steps - 1][np / 2] closed quads of each step transition
+  std::vector<char> fuse;      // transition s -> s+1 runs as one fused update + Gram
 
   size_t keys() const { return (size_t)np * steps; }
 
@@ -128,6 +130,58 @@ struct Work {
     HIP_CALL(hipMemcpy(d_skip, z.data(), keys(), hipMemcpyHostToDevice));
     memset(h_ctl, 0, sizeof(RunCtl));
     HIP_CALL(hipMemcpy(d_ctl, h_ctl, sizeof(RunCtl), hipMemcpyHostToDevice));
+    build_quads();
+  }
+
+  // A transition s -> s+1 is fused when every pair of step s, with the pair
+  // holding its first block's next partner, forms a closed quad.
+  void build_quads()
+  {
+    fuse.assign(steps, 0);
+    if (np < 2u || (np & 1u) || steps < 2u || getenv("HJ_NOFUSE")) return;
+    std::vector<Quad> all((size_t)(steps - 1u) * (np / 2u));
+    const unsigned nb = 2u * np;
+    std::vector<int> slotA(nb), slotB(nb), partB(nb);
+    for (unsigned t = 0u; t + 1u < steps; ++t) {
+      for (unsigned p = 0u; p < np; ++p) {
+        slotA[strat1[t][p][0]] = (int)p;
+        slotA[strat1[t][p][1]] = (int)p;
+        const unsigned x = strat1[t + 1u][p][0], y = strat1[t + 1u][p][1];
+        slotB[x] = (int)p;
+        slotB[y] = (int)p;
+        partB[x] = (int)y;
+        partB[y] = (int)x;
+      }
+      std::vector<char> done(np, 0);
+      unsigned nq = 0u;
+      bool ok = true;
+      for (unsigned p = 0u; p < np && ok; ++p) {
+        if (done[p]) continue;
+        const unsigned a = strat1[t][p][0], b = strat1[t][p][1];
+        const unsigned p1 = (unsigned)slotA[partB[a]];
+        if (p1 == p || done[p1]) { ok = false; break; }
+        const unsigned c = strat1[t][p1][0], d = strat1[t][p1][1];
+        const unsigned pb = (unsigned)partB[b];
+        if (!((pb == c || pb == d) && pb != (unsigned)partB[a])) { ok = false; break; }
+        auto tile = [&](unsigned blk) -> uint8_t {
+          return (uint8_t)((blk == a) ? 0u : (blk == b) ? 1u : (blk == c) ? 2u : 3u);
+        };
+        Quad q;
+        q.p0 = (uint16_t)p;
+        q.p1 = (uint16_t)p1;
+        q.n0 = (uint16_t)slotB[a];
+        q.n1 = (uint16_t)slotB[b];
+        q.x0 = tile(strat1[t + 1u][q.n0][0]);
+        q.y0 = tile(strat1[t + 1u][q.n0][1]);
+        q.x1 = tile(strat1[t + 1u][q.n1][0]);
+        q.y1 = tile(strat1[t + 1u][q.n1][1]);
+        done[p] = done[p1] = 1;
+        all[(size_t)t * (np / 2u) + nq++] = q;
+      }
+      fuse[t] = (ok && nq == np / 2u) ? 1 : 0;
+    }
+    HIP_CALL(hipMalloc(&d_quads, sizeof(Quad) * all.size()));
+    HIP_CALL(hipMemcpy(d_quads, all.data(), sizeof(Quad) * all.size(), hipMemcpyHostToDevice));
   }
   void release()
   {
@@ -141,6 +195,7 @@ struct Work {
     HIP_CALL(hipFree(d_rec));
     HIP_CALL(hipFree(d_skip));
     HIP_CALL(hipFree(d_model));
+    if (d_quads) HIP_CALL(hipFree(d_quads));
   }
 };
 
@@ -171,6 +226,56 @@ static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, u
   if (ev) HIP_CALL(hipEventRecord(ev[2], s));
   launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
   if (ev) HIP_CALL(hipEventRecord(ev[3], s));
+}
+
+// One level-2 sweep: a standalone Gram where the previous transition was not
+// fused (always for step 0), the pair kernels, and for each fused
+// transition the update of step s together with the Gram of step s+1.
+// Profiling events per step: before Gram, before pair, before the update
+// (or the fused update + Gram), after.
+static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned nswp, int definite,
+                            unsigned nplus, double *dG, unsigned lddG, unsigned nrow, double *dV,
+                            unsigned lddV, unsigned ncol, hipEvent_t *evs, long *steps_left,
+                            bool planned, bool split, std::vector<char> *gram_launched)
+{
+  bool have_gram = false;
+  unsigned issued = 0u;
+  for (unsigned st = 0u; st < w.steps; ++st) {
+    if (steps_left) {
+      if (*steps_left <= 0) return issued;
+      --*steps_left;
+    }
+    ++issued;
+    if (gram_launched) (*gram_launched)[st] = have_gram ? 0 : 1;
+    hipEvent_t *ev = evs ? evs + (size_t)4u * st : nullptr;
+    const size_t off = (size_t)st * w.np;
+    const uint16_t *pairs = w.d_s1 + off * 2u;
+    const uint8_t *plan = planned ? w.d_skip + off : nullptr;
+    if (ev) HIP_CALL(hipEventRecord(ev[0], s));
+    if (!have_gram) {
+      launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, plan);
+      if (split)
+        launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, plan);
+    }
+    if (ev) HIP_CALL(hipEventRecord(ev[1], s));
+    launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
+                plan, w.d_s0, w.s0steps, nswp, definite, nplus);
+    if (ev) HIP_CALL(hipEventRecord(ev[2], s));
+    const bool fused = (st + 1u < w.steps) && w.fuse[st] &&
+                       !(steps_left && *steps_left <= 0);
+    if (fused) {
+      launch_upgram(s, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
+                    w.np / 2u, w.nchunk, w.CH, w.d_part,
+                    planned ? w.d_skip + off + w.np : nullptr);
+      if (dV)
+        launch_update(s, dG, lddG, 0u, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
+    }
+    else
+      launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
+    if (ev) HIP_CALL(hipEventRecord(ev[3], s));
+    have_gram = fused;
+  }
+  return issued;
 }
 
 // Sweeps + initD on device data (the body of 1/HypJacL2.cu:104-178).
@@ -238,9 +343,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     hipGraph_t graph;
     HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     sweep_head(s);
-    for (unsigned st = 0u; st < w.steps; ++st)
-      issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
-                 planned, split);
+    issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr, nullptr,
+                planned, split, nullptr);
     HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamEndCapture(s, &graph));
     HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -252,6 +356,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   unsigned blk_swp = 0u;
   long steps_done = 0;
   bool stop = false, certify = false;
+  unsigned issued = 0u;
+  std::vector<char> gram_launched(w.steps, 0);
   g_stats.skipped = 0ull;
   while (blk_swp < g_max_sweeps) {
     if (planned) {
@@ -273,23 +379,21 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     }
     if (eager) {
       sweep_head(s);
-      for (unsigned st = 0u; st < w.steps; ++st) {
-        if (g_step_limit > 0 && steps_done >= g_step_limit) {
-          stop = true;
-          break;
-        }
-        ++steps_done;
-        issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
-                   g_profile ? &evs[(size_t)4u * st] : nullptr, planned, split);
-      }
+      long left = (g_step_limit > 0) ? (g_step_limit - steps_done) : (long)w.steps;
+      issued = issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
+                           g_profile ? evs.data() : nullptr, &left, planned, split, &gram_launched);
+      steps_done += issued;
+      if (g_step_limit > 0 && steps_done >= g_step_limit)
+        stop = true;
       HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     }
     else
       HIP_CALL(hipGraphLaunch(gexec, s));
     HIP_CALL(hipStreamSynchronize(s));
     if (g_profile)
-      for (unsigned st = 0u; st < w.steps; ++st)
+      for (unsigned st = 0u; st < issued; ++st)
         for (int k = 0; k < 3; ++k) {
+          if (k == 0 && !gram_launched[st]) continue;
           float ms = 0.0f;
           HIP_CALL(hipEventElapsedTime(&ms, evs[(size_t)4u * st + k], evs[(size_t)4u * st + k + 1]));
           g_stats.kms[k] += ms;

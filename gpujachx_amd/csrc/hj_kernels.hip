@@ -34,12 +34,11 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 // Gram partials.  Workgroup (pair pi, chunk c) covers rows [c*CH, (c+1)*CH)
 // with nw waves; wave w takes the sub-chunk of CH/nw rows starting at
 // c*CH + w*CH/nw, and the nw wave results are added in wave order through
-// LDS.  Within a wave, lane l owns column l%16 of both blocks (k = l/16);
-// for a 64-row group, load j (j = 0..7) brings rows 8j+2k and 8j+2k+1, so
-// the four lanes of a column read 64 contiguous bytes per instruction, and
-// MFMA steps (j, e) take row 8j+2k+e in k-slot k.  The next group's loads are
-// in flight while this group's MFMAs run.  Tiles: 00 = block i with itself,
-// 10 = block j with block i, 11 = block j with itself.
+// LDS.  Within a wave, lane l owns column l%16 of both blocks and k-slot
+// k = l/16; MFMA step j of a 16-row slab takes row 4j + k -- the rows a lane
+// holds after the block update's MFMA, so the fused update feeds the same
+// sequence from registers.  Tiles: 00 = block i with itself, 10 = block j
+// with block i, 11 = block j with itself.
 __global__ __launch_bounds__(256, 2) void k_gram(const double *__restrict__ G, unsigned ldG,
                                               const uint16_t *__restrict__ pairs, unsigned nchunk,
                                               unsigned CH, double *__restrict__ part,
@@ -52,41 +51,37 @@ __global__ __launch_bounds__(256, 2) void k_gram(const double *__restrict__ G, u
     return;
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
   const unsigned SUB = CH / nw;
-  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 2u * (l >> 4);
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + (l >> 4);
   const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
   const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
-  // 32-row half groups (MFMA steps j = 4h .. 4h+3 of a 64-row group), the
-  // next half group in flight: half the registers of whole-group buffers,
-  // so twice the waves per SIMD to keep the FP64 MFMA pipe busy
-  double2 n0[4], n1[4];
+  // two 16-row slabs per iteration (value t: row 4t + k of the 32 rows),
+  // the next two in flight
+  double n0[8], n1[8];
 #pragma unroll
-  for (unsigned j = 0u; j < 4u; ++j) {
-    n0[j] = *reinterpret_cast<const double2 *>(g0 + 8u * j);
-    n1[j] = *reinterpret_cast<const double2 *>(g1 + 8u * j);
+  for (unsigned t = 0u; t < 8u; ++t) {
+    n0[t] = g0[4u * t];
+    n1[t] = g1[4u * t];
   }
   for (unsigned g = 0u; g < SUB; g += 32u) {
-    double2 x0[4], x1[4];
+    double x0[8], x1[8];
 #pragma unroll
-    for (unsigned j = 0u; j < 4u; ++j) {
-      x0[j] = n0[j];
-      x1[j] = n1[j];
+    for (unsigned t = 0u; t < 8u; ++t) {
+      x0[t] = n0[t];
+      x1[t] = n1[t];
     }
     if (g + 32u < SUB) {
 #pragma unroll
-      for (unsigned j = 0u; j < 4u; ++j) {
-        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 32u + 8u * j);
-        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 32u + 8u * j);
+      for (unsigned t = 0u; t < 8u; ++t) {
+        n0[t] = g0[g + 32u + 4u * t];
+        n1[t] = g1[g + 32u + 4u * t];
       }
     }
 #pragma unroll
-    for (unsigned j = 0u; j < 4u; ++j) {
-      a00 = mfma(x0[j].x, x0[j].x, a00);
-      a10 = mfma(x1[j].x, x0[j].x, a10);
-      a11 = mfma(x1[j].x, x1[j].x, a11);
-      a00 = mfma(x0[j].y, x0[j].y, a00);
-      a10 = mfma(x1[j].y, x0[j].y, a10);
-      a11 = mfma(x1[j].y, x1[j].y, a11);
+    for (unsigned t = 0u; t < 8u; ++t) {
+      a00 = mfma(x0[t], x0[t], a00);
+      a10 = mfma(x1[t], x0[t], a10);
+      a11 = mfma(x1[t], x1[t], a11);
     }
   }
   if (w) {
@@ -115,23 +110,20 @@ __global__ __launch_bounds__(256, 2) void k_gram(const double *__restrict__ G, u
 // Split-precision Gram partials (same grid, chunks, waves and output layout
 // as k_gram) for the pairs whose plan byte is PLAN_SPLIT.
 //
-// Per 32-row group a wave scales each column by a power of two (2^-e with
-// 2^(e-1) <= max |x| < 2^e over the column's 32 rows, so the scaled values
+// Per 16-row slab a wave scales each column by a power of two (2^-e with
+// 2^(e-1) <= max |x| < 2^e over the column's 16 rows, so the scaled values
 // lie in (-1, 1) whatever the FP64 range and however the column scales of a
-// block differ), rounds to FP32 and splits every value
-// into three bfloat16 pieces x = h + m + l (h = bf16(x), m = bf16(x - h),
-// l = bf16(x - h - m): the FP32 value exactly, up to the last piece's
-// rounding).  The products of a tile are summed on BF16 MFMA 16x16x32 with
-// FP32 accumulation in three magnitude classes, hh, hm + mh, hl + lh + mm
-// (the dropped terms ml, lm, ll are below 2^-24 relative), and after each
-// group the classes are added smallest first in FP64, unscaled exactly by
-// 2^(e_a + e_b) and added to the FP64 accumulators.  Relative error of an
-// entry, against sqrt(A_ii A_jj): about 2^-23 per group, FP64 across groups.
-//
-// Rows: lane l (column l%16, lane group q = l/16) holds rows 8q .. 8q+7 of
-// the 32-row group (MFMA k = 8q + t), 64 contiguous bytes per lane and 256
-// per column; one MFMA per class and tile per group.
-typedef __attribute__((ext_vector_type(8))) __bf16 bf8;
+// block differ), rounds to FP32 and splits every value into three bfloat16
+// pieces x = h + m + l (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m):
+// the FP32 value exactly, up to the last piece's rounding).  The products of
+// a tile run on BF16 MFMA 16x16x16 with FP32 accumulation in three magnitude
+// classes, hh, hm + mh, hl + lh + mm (the dropped terms ml, lm, ll are below
+// 2^-24 relative), and after each slab the classes are added smallest first
+// in FP64, unscaled exactly by 2^(e_row + e_col) and added to the FP64
+// accumulators.  Relative error of an entry, against sqrt(A_ii A_jj): about
+// 2^-23 per slab, FP64 across slabs.  Rows: lane l holds rows 4t + l/16 of
+// column l%16 in slot t, as the FP64 Gram and the block update hold them.
+typedef __attribute__((ext_vector_type(4))) short s4;
 typedef __attribute__((ext_vector_type(4))) float f4;
 
 // row of a 16x16 FP32 MFMA result held by lane l in register r
@@ -140,9 +132,12 @@ __device__ __forceinline__ unsigned f32_out_row(unsigned l, unsigned r)
   return F32_OUT_ROW(l, r);
 }
 
-__device__ __forceinline__ f4 mfma_bf(const bf8 &a, const bf8 &b, f4 c)
+// BF16 MFMA 16x16x16: lane l gives 4 values of column l%16 of each operand
+// (its rows 4t + l/16 of the slab, slot t); any one-to-one assignment of
+// rows to k serves a Gram, the same for both operands
+__device__ __forceinline__ f4 mfma_bf(const s4 &a, const s4 &b, f4 c)
 {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 // biased FP64 exponent field of x (0 for zero / subnormal)
@@ -159,19 +154,29 @@ __device__ __forceinline__ int column_max_int(int v)
   return v;
 }
 
-// three-way bf16 split of 8 FP64 values scaled by 2^-e
-__device__ __forceinline__ void split3(const double *x, int e, bf8 &h, bf8 &m, bf8 &lo)
+__device__ __forceinline__ short bf16_of(float v)
+{
+  return __builtin_bit_cast(short, (__bf16)v);
+}
+
+__device__ __forceinline__ float float_of_bf16(short b)
+{
+  return __uint_as_float(((unsigned)(unsigned short)b) << 16);
+}
+
+// three-way bf16 split of 4 FP64 values scaled by 2^-e
+__device__ __forceinline__ void split3(const d4 &x, int e, s4 &h, s4 &m, s4 &lo)
 {
 #pragma unroll
-  for (unsigned t = 0u; t < 8u; ++t) {
+  for (unsigned t = 0u; t < 4u; ++t) {
     const float v = (float)ldexp(x[t], -e);
-    const __bf16 a = (__bf16)v;
-    const float r1 = v - (float)a;
-    const __bf16 b = (__bf16)r1;
-    const float r2 = r1 - (float)b;
+    const short a = bf16_of(v);
+    const float r1 = v - float_of_bf16(a);
+    const short b = bf16_of(r1);
+    const float r2 = r1 - float_of_bf16(b);
     h[t] = a;
     m[t] = b;
-    lo[t] = (__bf16)r2;
+    lo[t] = bf16_of(r2);
   }
 }
 
@@ -179,8 +184,8 @@ struct Cls {
   f4 c0, c1, c2;  // hh, hm + mh, hl + lh + mm
 };
 
-__device__ __forceinline__ void tile_products(Cls &C, const bf8 &ha, const bf8 &ma, const bf8 &la,
-                                              const bf8 &hb, const bf8 &mb, const bf8 &lb)
+__device__ __forceinline__ void tile_products(Cls &C, const s4 &ha, const s4 &ma, const s4 &la,
+                                              const s4 &hb, const s4 &mb, const s4 &lb)
 {
   C.c0 = mfma_bf(ha, hb, C.c0);
   C.c1 = mfma_bf(ha, mb, C.c1);
@@ -192,17 +197,49 @@ __device__ __forceinline__ void tile_products(Cls &C, const bf8 &ha, const bf8 &
 
 // adds the classes of a tile to its FP64 accumulator, element r (row
 // 4(l/16) + r, column l%16) unscaled by 2^(erow[r] + ecol)
-__device__ __forceinline__ void fold(d4 &acc, Cls &C, const int *erow, int ecol)
+__device__ __forceinline__ void fold(d4 &acc, const Cls &C, const int *erow, int ecol)
 {
 #pragma unroll
   for (unsigned r = 0u; r < 4u; ++r) {
     double t = (double)C.c2[r] + (double)C.c1[r];
     t = t + (double)C.c0[r];
     acc[r] = acc[r] + ldexp(t, erow[r] + ecol);
-    C.c0[r] = 0.0f;
-    C.c1[r] = 0.0f;
-    C.c2[r] = 0.0f;
   }
+}
+
+// split Gram of one 16-row slab: X, Y hold the lane's rows 4t + l/16 of
+// column l%16 of the pair's first and second block
+__device__ __forceinline__ void gram_split_slab(const d4 &X, const d4 &Y, unsigned l, d4 &a00,
+                                                d4 &a10, d4 &a11)
+{
+  int e0 = 0, e1 = 0;
+#pragma unroll
+  for (unsigned t = 0u; t < 4u; ++t) {
+    e0 = max(e0, exp_field(X[t]));
+    e1 = max(e1, exp_field(Y[t]));
+  }
+  e0 = column_max_int(e0);
+  e1 = column_max_int(e1);
+  e0 = e0 ? (e0 - 1022) : 0;
+  e1 = e1 ? (e1 - 1022) : 0;
+  // exponents of the columns behind the rows this lane's results hold
+  int r0e[4], r1e[4];
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    r0e[r] = __shfl(e0, (int)f32_out_row(l, r), 64);
+    r1e[r] = __shfl(e1, (int)f32_out_row(l, r), 64);
+  }
+  s4 h0, m0, q0, h1, m1, q1;
+  split3(X, e0, h0, m0, q0);
+  split3(Y, e1, h1, m1, q1);
+  const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+  Cls C00 = {z, z, z}, C10 = {z, z, z}, C11 = {z, z, z};
+  tile_products(C00, h0, m0, q0, h0, m0, q0);
+  tile_products(C10, h1, m1, q1, h0, m0, q0);
+  tile_products(C11, h1, m1, q1, h1, m1, q1);
+  fold(a00, C00, r0e, e0);
+  fold(a10, C10, r1e, e0);
+  fold(a11, C11, r1e, e1);
 }
 
 __global__ __launch_bounds__(256, 2) void k_gram_split(const double *__restrict__ G, unsigned ldG,
@@ -218,62 +255,34 @@ __global__ __launch_bounds__(256, 2) void k_gram_split(const double *__restrict_
     return;
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
   const unsigned SUB = CH / nw;
-  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + 8u * (l >> 4);
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + (l >> 4);
   const double *const g0 = G + (size_t)(16u * iBlk + (l & 15u)) * ldG + r0;
   const double *const g1 = G + (size_t)(16u * jBlk + (l & 15u)) * ldG + r0;
   d4 a00 = {0.0, 0.0, 0.0, 0.0}, a10 = a00, a11 = a00;
-  Cls C00, C10, C11;
-  C00.c0 = C00.c1 = C00.c2 = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  C10 = C00;
-  C11 = C00;
-  double2 n0[4], n1[4];
+  double n0[8], n1[8];
 #pragma unroll
-  for (unsigned j = 0u; j < 4u; ++j) {
-    n0[j] = *reinterpret_cast<const double2 *>(g0 + 2u * j);
-    n1[j] = *reinterpret_cast<const double2 *>(g1 + 2u * j);
+  for (unsigned t = 0u; t < 8u; ++t) {
+    n0[t] = g0[4u * t];
+    n1[t] = g1[4u * t];
   }
   for (unsigned g = 0u; g < SUB; g += 32u) {
-    double x0[8], x1[8];
+    d4 X0, X1, Y0, Y1;
 #pragma unroll
-    for (unsigned j = 0u; j < 4u; ++j) {
-      x0[2u * j] = n0[j].x;
-      x0[2u * j + 1u] = n0[j].y;
-      x1[2u * j] = n1[j].x;
-      x1[2u * j + 1u] = n1[j].y;
+    for (unsigned t = 0u; t < 4u; ++t) {
+      X0[t] = n0[t];
+      X1[t] = n0[4u + t];
+      Y0[t] = n1[t];
+      Y1[t] = n1[4u + t];
     }
     if (g + 32u < SUB) {
 #pragma unroll
-      for (unsigned j = 0u; j < 4u; ++j) {
-        n0[j] = *reinterpret_cast<const double2 *>(g0 + g + 32u + 2u * j);
-        n1[j] = *reinterpret_cast<const double2 *>(g1 + g + 32u + 2u * j);
+      for (unsigned t = 0u; t < 8u; ++t) {
+        n0[t] = g0[g + 32u + 4u * t];
+        n1[t] = g1[g + 32u + 4u * t];
       }
     }
-    int e0 = 0, e1 = 0;
-#pragma unroll
-    for (unsigned t = 0u; t < 8u; ++t) {
-      e0 = max(e0, exp_field(x0[t]));
-      e1 = max(e1, exp_field(x1[t]));
-    }
-    e0 = column_max_int(e0);
-    e1 = column_max_int(e1);
-    e0 = e0 ? (e0 - 1022) : 0;
-    e1 = e1 ? (e1 - 1022) : 0;
-    // exponents of the columns behind the rows this lane's results hold
-    int r0e[4], r1e[4];
-#pragma unroll
-    for (unsigned r = 0u; r < 4u; ++r) {
-      r0e[r] = __shfl(e0, (int)f32_out_row(l, r), 64);
-      r1e[r] = __shfl(e1, (int)f32_out_row(l, r), 64);
-    }
-    bf8 h0, m0, q0, h1, m1, q1;
-    split3(x0, e0, h0, m0, q0);
-    split3(x1, e1, h1, m1, q1);
-    tile_products(C00, h0, m0, q0, h0, m0, q0);
-    tile_products(C10, h1, m1, q1, h0, m0, q0);
-    tile_products(C11, h1, m1, q1, h1, m1, q1);
-    fold(a00, C00, r0e, e0);
-    fold(a10, C10, r1e, e0);
-    fold(a11, C11, r1e, e1);
+    gram_split_slab(X0, Y0, l, a00, a10, a11);
+    gram_split_slab(X1, Y1, l, a00, a10, a11);
   }
   if (w) {
     red[0][w - 1u][l] = a00;
@@ -660,6 +669,177 @@ __global__ __launch_bounds__(64) void k_update(double *__restrict__ G, unsigned 
 // Column norms, G -> U, D = +-||g||^2 (share/device_code_common.hpp:187-219).
 // One wave per column: lane l sums rows l, l+64, ... by fma, then an xor
 // butterfly over 64 lanes (32,16,8,4,2,1).
+// ---------------------------------------------------------------------------
+// Fused step transition: the block update of step s for the two pairs of a
+// closed quad and the Gram partials of the two pairs of step s+1 they form,
+// from the updated values in registers (one HBM read and one write of the
+// four blocks instead of the update's read + write and the next Gram's
+// read).  Grid: nquads x nchunk workgroups of nw waves, the Gram's chunks
+// and waves (gram_waves), so the partials are bit for bit those of k_gram /
+// k_gram_split on the updated matrix.
+//
+// Update orientation D = G W (A = G, B = W): lane l, register r of the
+// result holds row 4r + l/16 of the 16-row slab, column l%16 of the tile --
+// exactly the Gram's operand; the products and their chain over k = 0..31
+// are those of k_update (operands swapped: the same fma results).
+
+// tile i (uniform) of four, selected by value
+__device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
+{
+  d4 o;
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    const double ab = (i & 1u) ? b[r] : a[r];
+    const double cd = (i & 1u) ? d[r] : c[r];
+    o[r] = (i & 2u) ? cd : ab;
+  }
+  return o;
+}
+
+#define LDW 33u  // padded row length of the W copies in LDS
+
+// one 16-row slab of [G_a G_b] W, W(k, j) = Wl[k * LDW + j] in LDS (or the
+// unchanged slab if !upd): t0, t1 in the Gram layout; writes the updated slab
+__device__ __forceinline__ void slab_update(double *__restrict__ G, unsigned ldG, size_t row,
+                                            unsigned a, unsigned b, bool upd, const double *Wl,
+                                            unsigned l, d4 &t0, d4 &t1)
+{
+  if (upd) {
+    double x[8];
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u) {
+      const unsigned cc = 4u * u + (l >> 4);
+      const unsigned gc = (cc < 16u) ? (16u * a + cc) : (16u * b + cc - 16u);
+      x[u] = G[(size_t)gc * ldG + row + (l & 15u)];
+    }
+    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u) {
+      const double *const wr = Wl + (4u * u + (l >> 4)) * LDW + (l & 15u);
+      acc0 = mfma(x[u], wr[0], acc0);
+      acc1 = mfma(x[u], wr[16], acc1);
+    }
+    double *const oa = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
+    double *const ob = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
+#pragma unroll
+    for (unsigned r = 0u; r < 4u; ++r) {
+      oa[4u * r] = acc0[r];
+      ob[4u * r] = acc1[r];
+    }
+    t0 = acc0;
+    t1 = acc1;
+  }
+  else {
+    const double *const ia = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
+    const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
+#pragma unroll
+    for (unsigned r = 0u; r < 4u; ++r) {
+      t0[r] = ia[4u * r];
+      t1[r] = ib[4u * r];
+    }
+  }
+}
+
+__device__ __forceinline__ void gram_fp64_slab(const d4 &X, const d4 &Y, d4 &a00, d4 &a10, d4 &a11)
+{
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    a00 = mfma(X[r], X[r], a00);
+    a10 = mfma(Y[r], X[r], a10);
+    a11 = mfma(Y[r], Y[r], a11);
+  }
+}
+
+// stores a wave-reduced Gram partial: FP64 tiles at l*4 + r as k_gram does,
+// split tiles (FP32 MFMA layout) the way k_gram_split stores them
+__device__ __forceinline__ void store_partial(double *o, unsigned l, bool split, const d4 &a00,
+                                              const d4 &a10, const d4 &a11)
+{
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    unsigned idx = l * 4u + r;
+    if (split) {
+      const unsigned i = f32_out_row(l, r);
+      idx = ((l & 15u) + 16u * (i & 3u)) * 4u + (i >> 2);
+    }
+    o[idx] = a00[r];
+    o[256u + idx] = a10[r];
+    o[512u + idx] = a11[r];
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_upgram(double *__restrict__ G, unsigned ldG,
+                                                const uint16_t *__restrict__ pairs,
+                                                const double *__restrict__ Wbuf,
+                                                const unsigned *__restrict__ flags,
+                                                const Quad *__restrict__ quads, unsigned nchunk,
+                                                unsigned CH, double *__restrict__ part,
+                                                const uint8_t *__restrict__ plan_next)
+{
+  __shared__ d4 red[2][3][GRAM_WAVES - 1][64];
+  __shared__ double Wl[2][32u * LDW];
+  const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned qi = blockIdx.x / nchunk, c = blockIdx.x - qi * nchunk;
+  const Quad Q = quads[qi];
+  const unsigned A0 = pairs[2u * Q.p0], B0 = pairs[2u * Q.p0 + 1u];
+  const unsigned A1 = pairs[2u * Q.p1], B1 = pairs[2u * Q.p1 + 1u];
+  const bool u0 = flags[Q.p0] != 0u, u1 = flags[Q.p1] != 0u;
+  const uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
+  const uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
+  // W(k, j) of both pairs into LDS (row-major, padded)
+  for (unsigned e = threadIdx.x; e < 2048u; e += blockDim.x) {
+    const unsigned pp = e >> 10, k = (e >> 5) & 31u, j = e & 31u;
+    Wl[pp][k * LDW + j] = F32(Wbuf + (size_t)(pp ? Q.p1 : Q.p0) * 1024u, k, j);
+  }
+  __syncthreads();
+  const unsigned SUB = CH / nw;
+  const size_t rbeg = (size_t)c * CH + (size_t)w * SUB;
+  d4 z = {0.0, 0.0, 0.0, 0.0};
+  d4 g00 = z, g10 = z, g11 = z, h00 = z, h10 = z, h11 = z;
+  for (unsigned g = 0u; g < SUB; g += 16u) {
+    d4 t0, t1, t2, t3;
+    slab_update(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, t0, t1);
+    slab_update(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, t2, t3);
+    if (pn0 != PLAN_SKIP) {
+      const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
+      if (pn0 == PLAN_FP64)
+        gram_fp64_slab(X, Y, g00, g10, g11);
+      else
+        gram_split_slab(X, Y, l, g00, g10, g11);
+    }
+    if (pn1 != PLAN_SKIP) {
+      const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
+      if (pn1 == PLAN_FP64)
+        gram_fp64_slab(X, Y, h00, h10, h11);
+      else
+        gram_split_slab(X, Y, l, h00, h10, h11);
+    }
+  }
+  if (w) {
+    red[0][0][w - 1u][l] = g00;
+    red[0][1][w - 1u][l] = g10;
+    red[0][2][w - 1u][l] = g11;
+    red[1][0][w - 1u][l] = h00;
+    red[1][1][w - 1u][l] = h10;
+    red[1][2][w - 1u][l] = h11;
+  }
+  __syncthreads();
+  if (w == 0u) {
+    for (unsigned v = 1u; v < nw; ++v) {
+      g00 = g00 + red[0][0][v - 1u][l];
+      g10 = g10 + red[0][1][v - 1u][l];
+      g11 = g11 + red[0][2][v - 1u][l];
+      h00 = h00 + red[1][0][v - 1u][l];
+      h10 = h10 + red[1][1][v - 1u][l];
+      h11 = h11 + red[1][2][v - 1u][l];
+    }
+    if (pn0 != PLAN_SKIP)
+      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, pn0 == PLAN_SPLIT, g00, g10, g11);
+    if (pn1 != PLAN_SKIP)
+      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, pn1 == PLAN_SPLIT, h00, h10, h11);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_initD(double *__restrict__ G, double *__restrict__ D,
                                                unsigned ifc, unsigned nRow, unsigned nRank,
                                                unsigned nPlus, unsigned ldG)
@@ -739,6 +919,15 @@ void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *p
   const unsigned nw = gram_waves(CH);
   hipLaunchKernelGGL(k_gram, dim3(np * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, nchunk, CH, part,
                      skip);
+}
+
+void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
+                   const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
+                   unsigned CH, double *part, const uint8_t *plan_next)
+{
+  const unsigned nw = gram_waves(CH);
+  hipLaunchKernelGGL(k_upgram, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W, flags,
+                     quads, nchunk, CH, part, plan_next);
 }
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,

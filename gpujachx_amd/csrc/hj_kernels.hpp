@@ -60,6 +60,18 @@ inline unsigned gram_waves(unsigned CH)
 
 void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs, unsigned np,
                  unsigned nchunk, unsigned CH, double *part, const uint8_t *skip);
+// A closed quad of a step transition s -> s+1: two pairs of step s whose
+// four blocks form two pairs of step s+1.  Tiles 0..3 are the first and
+// second block of pair p0, then of pair p1.
+struct Quad {
+  uint16_t p0, p1;          // slots of the current pairs in step s
+  uint16_t n0, n1;          // slots of the next pairs in step s+1
+  uint8_t x0, y0, x1, y1;   // tiles of the first / second block of n0 and n1
+};
+
+void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
+                   const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
+                   unsigned CH, double *part, const uint8_t *plan_next);
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
                        unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,

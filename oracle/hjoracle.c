@@ -193,11 +193,10 @@ static void gram_wave(const double *G0, const double *G1, unsigned ld,
   memset(a00, 0, sizeof(a00));
   memset(a10, 0, sizeof(a10));
   memset(a11, 0, sizeof(a11));
-  for (unsigned g = 0u; g < nr / 64u; ++g)
-    for (unsigned j = 0u; j < 8u; ++j)
-      for (unsigned e = 0u; e < 2u; ++e)
+  for (unsigned g = 0u; g < nr / 16u; ++g)
+    for (unsigned j = 0u; j < 4u; ++j)
         for (unsigned k = 0u; k < 4u; ++k) {
-        const size_t row = (size_t)r0 + 64u * g + 8u * j + 2u * k + e;
+        const size_t row = (size_t)r0 + 16u * g + 4u * j + k;
         double x0[16], x1[16];
         for (unsigned i = 0u; i < 16u; ++i) {
           x0[i] = G0[(size_t)i * ld + row];

@@ -11,11 +11,12 @@
  * the GPU path is expected to match this oracle bit for bit:
  *   - Gram partial of a row chunk: split into hjo_gram_waves() equal
  *     sub-chunks, one per wave, whose results are added in wave order; in a
- *     sub-chunk, 64-row groups ascending; inside a group, 16 MFMA steps
- *     (j = 0..7, e = 0..1), each a chain over k = 0..3 of fma(x,y,acc) on
- *     row 64g+8j+2k+e (the gfx950 FP64 MFMA 16x16x4 is a sequential fma
- *     chain in k, measured on the device); partials of the chunks are added
- *     in order.
+ *     sub-chunk, 16-row slabs ascending; inside a slab, 4 MFMA steps
+ *     (j = 0..3), each a chain over k = 0..3 of fma(x,y,acc) on row
+ *     16g+4j+k (the gfx950 FP64 MFMA 16x16x4 is a sequential fma chain in k,
+ *     measured on the device; row 4j+k of a slab is where the block
+ *     update's MFMA leaves it, so the update can feed the next Gram from
+ *     registers); partials of the chunks are added in order.
  *   - Block update: out(r,j) = fma chain over k=0..31 of G(r,k)*W(k,j) from +0.
  *   - Column norms in initD: 64 lane-strided fma chains + xor butterfly over
  *     64 lanes (32,16,8,4,2,1).

@@ -105,16 +105,19 @@ def test_learned_plus_split_matches_plain():
     if not os.path.exists(lr.DEFAULT_MODEL):
         pytest.skip("no trained model file")
     G = svd_input(1024, 1024, 10.0 ** (-12.0 * np.random.default_rng(2).uniform(size=1024)), seed=43)
-    rp, _, _, _ = _run(g.FORM_PLAIN, 9, G, 1024)
-    g.init_strats_builtin("cycwor", 32, "cycwor", 64)
-    lr.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
+    old = g.set_max_sweeps(80)
     try:
+        rp, _, _, rbp = _run(g.FORM_PLAIN, 9, G, 1024)
+        g.init_strats_builtin("cycwor", 32, "cycwor", 64)
+        lr.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
         rl = g.hypJacL2(9, G, 1024)
         sp, _ = g.last_splits()
         sk, _, _ = g.last_skips()
         _, rb, _ = g.last_sweeps()
     finally:
         g.set_form(g.FORM_PLAIN)
+        g.set_max_sweeps(old)
+    assert rbp[-1] == 0
     assert rb[-1] == 0 and sp[-1] == 0 and sk[-1] == 0
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()
