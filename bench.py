@@ -202,9 +202,12 @@ def main():
     rel = np.abs(sig_s - ex_s) / ex_s
     normwise = np.max(np.abs(sig_s - ex_s)) / ex_s[0]
 
-    # roofline of the dominant kernel (the block update) and of the Gram:
-    # algorithmic bytes per launch = executed pair visits per launch x bytes
-    # per visit (read + write of the m x 32 pair: 512 m; Gram: read, 256 m)
+    # roofline of the dominant kernel, the block update of a step fused with
+    # the next step's Gram (k_upgram; the last step of a sweep runs the plain
+    # update): algorithmic bytes per launch = executed pair visits per launch
+    # x bytes per visit (read + write of the m x 32 pair: 512 m); flops per
+    # visit 2048 m (update) + 1536 m (fused Gram).  The standalone Gram runs
+    # once per sweep (step 0): 256 m bytes per visit.
     np_ = n // 32
     steps_per_sweep = n // 16 - 1
     pairs_per_launch = main_run["visits"] / max(kcnt[2], 1)
@@ -215,7 +218,7 @@ def main():
     pair_ms = kms[1] / max(kcnt[1], 1)
     ach = upd_bytes / (upd_ms * 1e-3) / 1e9
     gram_ach = gram_bytes / (gram_ms * 1e-3) / 1e9
-    upd_tf = pairs_per_launch * 2048.0 * n / (upd_ms * 1e-3) / 1e12
+    upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     gram_tf = pairs_per_launch * 1536.0 * n / (gram_ms * 1e-3) / 1e12
 
     traffic = None
@@ -257,7 +260,7 @@ def main():
                       "frac_rel_le_1e-12": float(np.mean(rel <= 1e-12))},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_update", "avg_launch_ms": upd_ms,
+                     "kernel": "k_upgram (block update + next-step Gram)", "avg_launch_ms": upd_ms,
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},
         "kernels": {"gram": {"avg_ms": gram_ms, "GBs": gram_ach, "tflops": gram_tf},

@@ -187,11 +187,6 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
   }
 
   wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
-#pragma unroll
-  for (unsigned e = 0u; e < 16u; ++e) {
-    const unsigned k = l + 64u * e;
-    BR(W, k & 31u, k >> 5) = ((k & 31u) == (k >> 5)) ? 1.0 : 0.0;
-  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -203,25 +198,21 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
 #pragma unroll
     for (unsigned j = 0u; j < 32u; ++j)
       a[j] = (j <= x) ? BR(R, x, j) : 0.0;
-#pragma unroll
-    for (unsigned k = 0u; k < 32u; ++k) {
-      const double akk = readlane_d(a[k], k);
-      const double rk = dev_rsqrt(akk);
-      const double dk = sqrt(akk);
-      if (x > k) a[k] = a[k] * rk;
-      else if (x == k) a[k] = dk;
-#pragma unroll
-      for (unsigned j = k + 1u; j < 32u; ++j) {
-        const double ljk = readlane_d(a[k], j);
-        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
-      }
-    }
+    chol32_rows(a, x, W);  // W is free until the identity is written below
     __builtin_amdgcn_wave_barrier();
     if (l < 32u) {
 #pragma unroll
       for (unsigned y = 0u; y < 32u; ++y)
         BR(R, y, x) = (y <= x) ? a[y] : 0.0;
     }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (unsigned e = 0u; e < 16u; ++e) {
+    const unsigned k = l + 64u * e;
+    BR(W, k & 31u, k >> 5) = ((k & 31u) == (k >> 5)) ? 1.0 : 0.0;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

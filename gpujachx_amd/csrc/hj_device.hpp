@@ -174,6 +174,39 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
   return __hiloint2double(hi, lo);
 }
 
+// In-place 32x32 Cholesky by one wave, share/device_code_common_Cholesky.hpp:58-158
+// (USE_RSQRT): lane x (lanes 32..63 duplicate lanes 0..31) holds row x of the
+// lower triangle, a[j] = A(x, j) for j <= x, else 0; column k is scaled by
+// rsqrt(A(k,k)), the diagonal set to sqrt(A(k,k)), then
+// A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k.  On return a[j] =
+// L(x, j).  The pivot comes by readlane; column k of L goes through the
+// 32-double LDS buffer col (broadcast reads, no scalar registers held).
+__device__ __forceinline__ void chol32_rows(double (&a)[32], unsigned x, double *col)
+{
+#pragma unroll
+  for (unsigned k = 0u; k < 32u; ++k) {
+    const double akk = readlane_d(a[k], k);
+    const double rk = dev_rsqrt(akk);
+    const double dk = sqrt(akk);
+    if (x > k) a[k] = a[k] * rk;
+    else if (x == k) a[k] = dk;
+    if (k + 1u < 32u) {
+      col[x] = a[k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (unsigned j = k + 1u; j < 32u; ++j) {
+        const double ljk = col[j];
+        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+}
+
 }  // namespace hj
 
 #endif

@@ -264,7 +264,7 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
     const bool fused = (st + 1u < w.steps) && w.fuse[st] &&
                        !(steps_left && *steps_left <= 0);
     if (fused) {
-      launch_upgram(s, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
+      launch_upgram(s, split, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
                     w.np / 2u, w.nchunk, w.CH, w.d_part,
                     planned ? w.d_skip + off + w.np : nullptr);
       if (dV)

@@ -339,6 +339,7 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
   __shared__ float cmax[16];
   __shared__ float bnorm[2];
   __shared__ int chol_ok;
+  __shared__ double cholcol[32];
 
   const unsigned tid = threadIdx.x;
   const unsigned pi = blockIdx.x;
@@ -408,19 +409,7 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
 #pragma unroll
     for (unsigned j = 0u; j < 32u; ++j)
       a[j] = (j <= x) ? RA(R, x, j) : 0.0;
-#pragma unroll
-    for (unsigned k = 0u; k < 32u; ++k) {
-      const double akk = readlane_d(a[k], k);
-      const double rk = dev_rsqrt(akk);
-      const double dk = sqrt(akk);
-      if (x > k) a[k] = a[k] * rk;
-      else if (x == k) a[k] = dk;
-#pragma unroll
-      for (unsigned j = k + 1u; j < 32u; ++j) {
-        const double ljk = readlane_d(a[k], j);
-        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
-      }
-    }
+    chol32_rows(a, x, cholcol);
     if (tid < 32u) {
 #pragma unroll
       for (unsigned y = 0u; y < 32u; ++y)
@@ -697,6 +686,13 @@ __device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
 }
 
 #define LDW 33u  // padded row length of the W copies in LDS
+// waves per SIMD the fused kernel is compiled for (register budget 512/n)
+#ifndef UPGRAM_WAVES_FP64
+#define UPGRAM_WAVES_FP64 3
+#endif
+#ifndef UPGRAM_WAVES_SPLIT
+#define UPGRAM_WAVES_SPLIT 2
+#endif
 
 // one 16-row slab of [G_a G_b] W, W(k, j) = Wl[k * LDW + j] in LDS (or the
 // unchanged slab if !upd): t0, t1 in the Gram layout; writes the updated slab
@@ -768,7 +764,11 @@ __device__ __forceinline__ void store_partial(double *o, unsigned l, bool split,
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_upgram(double *__restrict__ G, unsigned ldG,
+// SPLIT: the next pairs may use the split Gram (else every non-skipped next
+// pair gets the FP64 Gram, and the kernel needs far fewer registers)
+template <bool SPLIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPLIT ? UPGRAM_WAVES_SPLIT : UPGRAM_WAVES_FP64)))
+void k_upgram(double *__restrict__ G, unsigned ldG,
                                                 const uint16_t *__restrict__ pairs,
                                                 const double *__restrict__ Wbuf,
                                                 const unsigned *__restrict__ flags,
@@ -784,8 +784,14 @@ __global__ __launch_bounds__(256, 2) void k_upgram(double *__restrict__ G, unsig
   const unsigned A0 = pairs[2u * Q.p0], B0 = pairs[2u * Q.p0 + 1u];
   const unsigned A1 = pairs[2u * Q.p1], B1 = pairs[2u * Q.p1 + 1u];
   const bool u0 = flags[Q.p0] != 0u, u1 = flags[Q.p1] != 0u;
-  const uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
-  const uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
+  uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
+  uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
+  if (!SPLIT) {
+    pn0 = (pn0 == PLAN_SKIP) ? PLAN_SKIP : PLAN_FP64;
+    pn1 = (pn1 == PLAN_SKIP) ? PLAN_SKIP : PLAN_FP64;
+  }
+  if (!u0 && !u1 && pn0 == PLAN_SKIP && pn1 == PLAN_SKIP)
+    return;  // nothing to update, no Gram wanted
   // W(k, j) of both pairs into LDS (row-major, padded)
   for (unsigned e = threadIdx.x; e < 2048u; e += blockDim.x) {
     const unsigned pp = e >> 10, k = (e >> 5) & 31u, j = e & 31u;
@@ -802,14 +808,14 @@ __global__ __launch_bounds__(256, 2) void k_upgram(double *__restrict__ G, unsig
     slab_update(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, t2, t3);
     if (pn0 != PLAN_SKIP) {
       const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
-      if (pn0 == PLAN_FP64)
+      if (!SPLIT || pn0 == PLAN_FP64)
         gram_fp64_slab(X, Y, g00, g10, g11);
       else
         gram_split_slab(X, Y, l, g00, g10, g11);
     }
     if (pn1 != PLAN_SKIP) {
       const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
-      if (pn1 == PLAN_FP64)
+      if (!SPLIT || pn1 == PLAN_FP64)
         gram_fp64_slab(X, Y, h00, h10, h11);
       else
         gram_split_slab(X, Y, l, h00, h10, h11);
@@ -834,9 +840,11 @@ __global__ __launch_bounds__(256, 2) void k_upgram(double *__restrict__ G, unsig
       h11 = h11 + red[1][2][v - 1u][l];
     }
     if (pn0 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, pn0 == PLAN_SPLIT, g00, g10, g11);
+      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, SPLIT && pn0 == PLAN_SPLIT, g00, g10,
+                    g11);
     if (pn1 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, pn1 == PLAN_SPLIT, h00, h10, h11);
+      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, SPLIT && pn1 == PLAN_SPLIT, h00, h10,
+                    h11);
   }
 }
 
@@ -921,13 +929,17 @@ void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *p
                      skip);
 }
 
-void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
-                   const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
-                   unsigned CH, double *part, const uint8_t *plan_next)
+void launch_upgram(hipStream_t s, bool split, double *G, unsigned ldG, const uint16_t *pairs,
+                   const double *W, const unsigned *flags, const Quad *quads, unsigned nquads,
+                   unsigned nchunk, unsigned CH, double *part, const uint8_t *plan_next)
 {
   const unsigned nw = gram_waves(CH);
-  hipLaunchKernelGGL(k_upgram, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W, flags,
-                     quads, nchunk, CH, part, plan_next);
+  if (split)
+    hipLaunchKernelGGL(k_upgram<true>, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W,
+                       flags, quads, nchunk, CH, part, plan_next);
+  else
+    hipLaunchKernelGGL(k_upgram<false>, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W,
+                       flags, quads, nchunk, CH, part, plan_next);
 }
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
