@@ -13,7 +13,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libhypjac.so")
+# HJ_LIBHYPJAC: path of another build of the same library
+LIB_PATH = os.environ.get("HJ_LIBHYPJAC") or os.path.join(LIB_DIR, "libhypjac.so")
 STRAT_SO = os.path.join(LIB_DIR, "strat.so")
 
 # ALG bits (share/HypJacL.hpp:88-110)

@@ -133,6 +133,7 @@ struct Work {
     build_quads();
   }
 
+
   // A transition s -> s+1 is fused when every pair of step s, with the pair
   // holding its first block's next partner, forms a closed quad.
   void build_quads()
@@ -264,7 +265,7 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
     const bool fused = (st + 1u < w.steps) && w.fuse[st] &&
                        !(steps_left && *steps_left <= 0);
     if (fused) {
-      launch_upgram(s, split, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
+      launch_upgram(s, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
                     w.np / 2u, w.nchunk, w.CH, w.d_part,
                     planned ? w.d_skip + off + w.np : nullptr);
       if (dV)

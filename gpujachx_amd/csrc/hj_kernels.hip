@@ -375,6 +375,7 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
   // sum of the chunk partials in chunk order (all loads issued first)
   const double *const pp = part + (size_t)pi * nchunk * 768u;
   for (unsigned e = tid; e < 768u; e += 128u) {
+    const unsigned tile = e >> 8;
     double v[MAX_GRAM_CHUNKS];
 #pragma unroll
     for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
 #pragma unroll
     for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
       if (c < nchunk) s = s + v[c];
-    const unsigned tile = e >> 8, l = (e >> 2) & 63u, r = e & 3u;
+    const unsigned l = (e >> 2) & 63u, r = e & 3u;
     const unsigned i = 4u * r + (l >> 4), j = l & 15u;
     const unsigned row = (tile == 0u) ? i : (16u + i);
     const unsigned col = (tile == 2u) ? (16u + j) : j;
@@ -687,27 +688,83 @@ __device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
 
 #define LDW 33u  // padded row length of the W copies in LDS
 // waves per SIMD the fused kernel is compiled for (register budget 512/n)
-#ifndef UPGRAM_WAVES_FP64
-#define UPGRAM_WAVES_FP64 3
+#ifndef UPGRAM_WAVES
+#define UPGRAM_WAVES 2
 #endif
-#ifndef UPGRAM_WAVES_SPLIT
-#define UPGRAM_WAVES_SPLIT 2
+#ifndef UPGRAM_PREFETCH
+#define UPGRAM_PREFETCH 1
 #endif
 
-// one 16-row slab of [G_a G_b] W, W(k, j) = Wl[k * LDW + j] in LDS (or the
-// unchanged slab if !upd): t0, t1 in the Gram layout; writes the updated slab
-__device__ __forceinline__ void slab_update(double *__restrict__ G, unsigned ldG, size_t row,
-                                            unsigned a, unsigned b, bool upd, const double *Wl,
-                                            unsigned l, d4 &t0, d4 &t1)
+// Operands of one 16-row slab of a pair: for an updated pair the A operands
+// of [G_a G_b] W (lane l: row l%16 of pair column 4u + l/16), else the
+// unchanged slab in the Gram layout (x[r], x[4 + r]: row 4r + l/16 of
+// column l%16 of blocks a, b).
+__device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned ldG, size_t row,
+                                          unsigned a, unsigned b, bool upd, unsigned l, double *x)
 {
   if (upd) {
-    double x[8];
 #pragma unroll
     for (unsigned u = 0u; u < 8u; ++u) {
       const unsigned cc = 4u * u + (l >> 4);
       const unsigned gc = (cc < 16u) ? (16u * a + cc) : (16u * b + cc - 16u);
       x[u] = G[(size_t)gc * ldG + row + (l & 15u)];
     }
+  }
+  else {
+    const double *const ia = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
+    const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
+#pragma unroll
+    for (unsigned r = 0u; r < 4u; ++r) {
+      x[r] = ia[4u * r];
+      x[4u + r] = ib[4u * r];
+    }
+  }
+}
+
+#ifndef UPGRAM_LDS_STORE
+#define UPGRAM_LDS_STORE 1
+#endif
+#define LDT 17u  // padded column length of a wave's 16x16 transpose buffer
+
+// stores a 16x16 tile held in the Gram layout (lane l, register r: row
+// 4r + l/16, column l%16) to columns 16 blk .. of G, rows row .. row+15: via
+// the wave's LDS buffer T the lanes write 16 consecutive rows (128 bytes) of
+// one column per instruction
+__device__ __forceinline__ void store_tile(double *__restrict__ G, unsigned ldG, size_t row,
+                                           unsigned blk, const d4 &t, unsigned l, double *T)
+{
+#if UPGRAM_LDS_STORE
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r)
+    T[(l & 15u) * LDT + 4u * r + (l >> 4)] = t[r];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double v[4];
+#pragma unroll
+  for (unsigned q = 0u; q < 4u; ++q)
+    v[q] = T[(4u * q + (l >> 4)) * LDT + (l & 15u)];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (unsigned q = 0u; q < 4u; ++q)
+    G[(size_t)(16u * blk + 4u * q + (l >> 4)) * ldG + row + (l & 15u)] = v[q];
+#else
+  double *const o = G + (size_t)(16u * blk + (l & 15u)) * ldG + row + (l >> 4);
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r)
+    o[4u * r] = t[r];
+#endif
+}
+
+// [G_a G_b] W of the slab, W(k, j) = Wl[k * LDW + j] in LDS, written back;
+// t0, t1: the slab of blocks a, b in the Gram layout
+__device__ __forceinline__ void slab_compute(double *__restrict__ G, unsigned ldG, size_t row,
+                                             unsigned a, unsigned b, bool upd, const double *Wl,
+                                             unsigned l, const double *x, d4 &t0, d4 &t1, double *T)
+{
+  if (upd) {
     d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
 #pragma unroll
     for (unsigned u = 0u; u < 8u; ++u) {
@@ -715,23 +772,16 @@ __device__ __forceinline__ void slab_update(double *__restrict__ G, unsigned ldG
       acc0 = mfma(x[u], wr[0], acc0);
       acc1 = mfma(x[u], wr[16], acc1);
     }
-    double *const oa = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
-    double *const ob = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
-#pragma unroll
-    for (unsigned r = 0u; r < 4u; ++r) {
-      oa[4u * r] = acc0[r];
-      ob[4u * r] = acc1[r];
-    }
+    store_tile(G, ldG, row, a, acc0, l, T);
+    store_tile(G, ldG, row, b, acc1, l, T);
     t0 = acc0;
     t1 = acc1;
   }
   else {
-    const double *const ia = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
-    const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
 #pragma unroll
     for (unsigned r = 0u; r < 4u; ++r) {
-      t0[r] = ia[4u * r];
-      t1[r] = ib[4u * r];
+      t0[r] = x[r];
+      t1[r] = x[4u + r];
     }
   }
 }
@@ -764,10 +814,10 @@ __device__ __forceinline__ void store_partial(double *o, unsigned l, bool split,
   }
 }
 
-// SPLIT: the next pairs may use the split Gram (else every non-skipped next
-// pair gets the FP64 Gram, and the kernel needs far fewer registers)
-template <bool SPLIT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPLIT ? UPGRAM_WAVES_SPLIT : UPGRAM_WAVES_FP64)))
+// Every non-skipped next pair gets the FP64 Gram, whatever its plan: riding
+// on the update's HBM stream the Gram adds no memory traffic, and this
+// kernel is bound by HBM, not by the FP64 MFMA units.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
 void k_upgram(double *__restrict__ G, unsigned ldG,
                                                 const uint16_t *__restrict__ pairs,
                                                 const double *__restrict__ Wbuf,
@@ -778,6 +828,7 @@ void k_upgram(double *__restrict__ G, unsigned ldG,
 {
   __shared__ d4 red[2][3][GRAM_WAVES - 1][64];
   __shared__ double Wl[2][32u * LDW];
+  __shared__ double Tb[GRAM_WAVES][16u * LDT];
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const unsigned qi = blockIdx.x / nchunk, c = blockIdx.x - qi * nchunk;
   const Quad Q = quads[qi];
@@ -786,10 +837,6 @@ void k_upgram(double *__restrict__ G, unsigned ldG,
   const bool u0 = flags[Q.p0] != 0u, u1 = flags[Q.p1] != 0u;
   uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
   uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
-  if (!SPLIT) {
-    pn0 = (pn0 == PLAN_SKIP) ? PLAN_SKIP : PLAN_FP64;
-    pn1 = (pn1 == PLAN_SKIP) ? PLAN_SKIP : PLAN_FP64;
-  }
   if (!u0 && !u1 && pn0 == PLAN_SKIP && pn1 == PLAN_SKIP)
     return;  // nothing to update, no Gram wanted
   // W(k, j) of both pairs into LDS (row-major, padded)
@@ -800,25 +847,42 @@ void k_upgram(double *__restrict__ G, unsigned ldG,
   __syncthreads();
   const unsigned SUB = CH / nw;
   const size_t rbeg = (size_t)c * CH + (size_t)w * SUB;
+  double *const Tw = Tb[w];
   d4 z = {0.0, 0.0, 0.0, 0.0};
   d4 g00 = z, g10 = z, g11 = z, h00 = z, h10 = z, h11 = z;
+  double xa[8], xb[8];
+#if UPGRAM_PREFETCH
+  slab_load(G, ldG, rbeg, A0, B0, u0, l, xa);
+  slab_load(G, ldG, rbeg, A1, B1, u1, l, xb);
+#endif
   for (unsigned g = 0u; g < SUB; g += 16u) {
     d4 t0, t1, t2, t3;
-    slab_update(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, t0, t1);
-    slab_update(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, t2, t3);
+#if UPGRAM_PREFETCH
+    double ca[8], cb[8];
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u) {
+      ca[u] = xa[u];
+      cb[u] = xb[u];
+    }
+    if (g + 16u < SUB) {
+      slab_load(G, ldG, rbeg + g + 16u, A0, B0, u0, l, xa);
+      slab_load(G, ldG, rbeg + g + 16u, A1, B1, u1, l, xb);
+    }
+    slab_compute(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, ca, t0, t1, Tw);
+    slab_compute(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, cb, t2, t3, Tw);
+#else
+    slab_load(G, ldG, rbeg + g, A0, B0, u0, l, xa);
+    slab_load(G, ldG, rbeg + g, A1, B1, u1, l, xb);
+    slab_compute(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, xa, t0, t1, Tw);
+    slab_compute(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, xb, t2, t3, Tw);
+#endif
     if (pn0 != PLAN_SKIP) {
       const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
-      if (!SPLIT || pn0 == PLAN_FP64)
-        gram_fp64_slab(X, Y, g00, g10, g11);
-      else
-        gram_split_slab(X, Y, l, g00, g10, g11);
+      gram_fp64_slab(X, Y, g00, g10, g11);
     }
     if (pn1 != PLAN_SKIP) {
       const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
-      if (!SPLIT || pn1 == PLAN_FP64)
-        gram_fp64_slab(X, Y, h00, h10, h11);
-      else
-        gram_split_slab(X, Y, l, h00, h10, h11);
+      gram_fp64_slab(X, Y, h00, h10, h11);
     }
   }
   if (w) {
@@ -840,11 +904,9 @@ void k_upgram(double *__restrict__ G, unsigned ldG,
       h11 = h11 + red[1][2][v - 1u][l];
     }
     if (pn0 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, SPLIT && pn0 == PLAN_SPLIT, g00, g10,
-                    g11);
+      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, false, g00, g10, g11);
     if (pn1 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, SPLIT && pn1 == PLAN_SPLIT, h00, h10,
-                    h11);
+      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, false, h00, h10, h11);
   }
 }
 
@@ -929,17 +991,13 @@ void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *p
                      skip);
 }
 
-void launch_upgram(hipStream_t s, bool split, double *G, unsigned ldG, const uint16_t *pairs,
-                   const double *W, const unsigned *flags, const Quad *quads, unsigned nquads,
-                   unsigned nchunk, unsigned CH, double *part, const uint8_t *plan_next)
+void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
+                   const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
+                   unsigned CH, double *part, const uint8_t *plan_next)
 {
   const unsigned nw = gram_waves(CH);
-  if (split)
-    hipLaunchKernelGGL(k_upgram<true>, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W,
-                       flags, quads, nchunk, CH, part, plan_next);
-  else
-    hipLaunchKernelGGL(k_upgram<false>, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W,
-                       flags, quads, nchunk, CH, part, plan_next);
+  hipLaunchKernelGGL(k_upgram, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W, flags,
+                     quads, nchunk, CH, part, plan_next);
 }
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,

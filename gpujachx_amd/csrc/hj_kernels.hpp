@@ -69,9 +69,9 @@ struct Quad {
   uint8_t x0, y0, x1, y1;   // tiles of the first / second block of n0 and n1
 };
 
-void launch_upgram(hipStream_t s, bool split, double *G, unsigned ldG, const uint16_t *pairs,
-                   const double *W, const unsigned *flags, const Quad *quads, unsigned nquads,
-                   unsigned nchunk, unsigned CH, double *part, const uint8_t *plan_next);
+void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
+                   const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
+                   unsigned CH, double *part, const uint8_t *plan_next);
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
                        unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
