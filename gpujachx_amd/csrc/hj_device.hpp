@@ -59,22 +59,25 @@ __device__ __forceinline__ double rsqrt_core(double m)
 // exact rescaling; IEEE special values as the reference returns them.
 __device__ __forceinline__ double dev_rsqrt(double x)
 {
-  uint64_t b = d2u(x);
-  int kadj = 0;
-  if ((b >> 52) - 1u >= 0x7feu) {  // not a positive normal number
-    if (x != x) return x + x;
-    if (x == 0.0) return u2d((b & 0x8000000000000000ull) | 0x7ff0000000000000ull);
-    if (x < 0.0) return u2d(0xfff8000000000000ull);
-    if (b == 0x7ff0000000000000ull) return 0.0;
-    x = x * 0x1p108;  // positive subnormal
-    b = d2u(x);
-    kadj = 54;
-  }
-  const int e = (int)(b >> 52) - 1023;
+  // special inputs resolved by selects (no branches: the callers' code stays
+  // straight-line for the scheduler); the main path also runs on them, its
+  // value discarded
+  const uint64_t b0 = d2u(x);
+  const bool special = ((b0 >> 52) - 1u) >= 0x7feu;  // not a positive normal number
+  const bool subn = special && (x > 0.0) && (b0 != 0x7ff0000000000000ull);
+  const double xs = subn ? x * 0x1p108 : x;  // positive subnormal: scaled into range
+  const int kadj = subn ? 54 : 0;
+  const uint64_t b = d2u(xs);
+  const int e = (int)((b >> 52) & 0x7ffu) - 1023;
   const int k = (e >= 0) ? (e / 2) : -((1 - e) / 2);
   const double m = u2d((b & 0x000fffffffffffffull) | ((uint64_t)(1023 + e - 2 * k) << 52));
   const double y = rsqrt_core(m);
-  return y * u2d((uint64_t)(1023 + kadj - k) << 52);
+  const double normal = y * u2d((uint64_t)(1023 + kadj - k) << 52);
+  double sp = 0.0;                                                   // +inf
+  sp = (x < 0.0) ? u2d(0xfff8000000000000ull) : sp;                  // negative: NaN
+  sp = (x == 0.0) ? u2d((b0 & 0x8000000000000000ull) | 0x7ff0000000000000ull) : sp;  // +-0: +-inf
+  sp = (x != x) ? x + x : sp;                                        // NaN
+  return (special && !subn) ? sp : normal;
 }
 
 // 2x2 rotation of share/device_code_common_rotate.hpp:189-237 (USE_RSQRT),
@@ -183,23 +186,33 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 // 32-double LDS buffer col (broadcast reads, no scalar registers held).
 __device__ __forceinline__ void chol32_rows(double (&a)[32], unsigned x, double *col)
 {
+  // pivot of column 0; later pivots are taken as soon as their own update is
+  // done, so their rsqrt / sqrt overlap the rest of the column's updates
+  double akk = readlane_d(a[0], 0);
+  double rk = dev_rsqrt(akk), dk = sqrt(akk);
 #pragma unroll
   for (unsigned k = 0u; k < 32u; ++k) {
-    const double akk = readlane_d(a[k], k);
-    const double rk = dev_rsqrt(akk);
-    const double dk = sqrt(akk);
     if (x > k) a[k] = a[k] * rk;
     else if (x == k) a[k] = dk;
     if (k + 1u < 32u) {
+      // the LDS accesses of a wavefront complete in program order, and col
+      // may alias itself, so the compiler keeps write-then-read order
       col[x] = a[k];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      {
+        const double l1 = col[k + 1u];
+        if (x >= k + 1u) a[k + 1u] = __fma_rn(-a[k], l1, a[k + 1u]);
+      }
+      akk = readlane_d(a[k + 1u], k + 1u);
+      const double rn = dev_rsqrt(akk), dn = sqrt(akk);
 #pragma unroll
-      for (unsigned j = k + 1u; j < 32u; ++j) {
+      for (unsigned j = k + 2u; j < 32u; ++j) {
         const double ljk = col[j];
         if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
       }
+      rk = rn;
+      dk = dn;
+      // keep the next column's broadcast loads below this point (register
+      // pressure)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -369,26 +369,36 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
     hyp = (def == 0);
   }
 
-  for (unsigned e = tid; e < s0steps * 32u; e += 128u)
-    S0[e] = strat0[e];
+  // the level-0 table (STRAT0_TAB_BYTES, 16-byte aligned) in one pass of
+  // 16-byte loads
+  if (tid < STRAT0_TAB_BYTES / 16u)
+    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(strat0)[tid];
 
-  // sum of the chunk partials in chunk order (all loads issued first)
+  // sum of the chunk partials in chunk order: three entries per thread at a
+  // time, all their chunk loads in flight together
   const double *const pp = part + (size_t)pi * nchunk * 768u;
-  for (unsigned e = tid; e < 768u; e += 128u) {
-    const unsigned tile = e >> 8;
-    double v[MAX_GRAM_CHUNKS];
+#pragma unroll 1
+  for (unsigned e0 = tid; e0 < 768u; e0 += 384u) {
+    double v[3][MAX_GRAM_CHUNKS];
 #pragma unroll
-    for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
-      if (c < nchunk) v[c] = pp[(size_t)c * 768u + e];
-    double s = v[0];
+    for (unsigned q = 0u; q < 3u; ++q)
+#pragma unroll
+      for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
+        if (c < nchunk) v[q][c] = pp[(size_t)c * 768u + e0 + 128u * q];
+#pragma unroll
+    for (unsigned q = 0u; q < 3u; ++q) {
+    const unsigned e = e0 + 128u * q;
+    const unsigned tile = e >> 8;
+    double s = v[q][0];
 #pragma unroll
     for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
-      if (c < nchunk) s = s + v[c];
+      if (c < nchunk) s = s + v[q][c];
     const unsigned l = (e >> 2) & 63u, r = e & 3u;
     const unsigned i = 4u * r + (l >> 4), j = l & 15u;
     const unsigned row = (tile == 0u) ? i : (16u + i);
     const unsigned col = (tile == 2u) ? (16u + j) : j;
     RA(R, row, col) = s;
+    }
   }
   for (unsigned e = tid; e < 1024u; e += 128u)
     RA(W, e & 31u, e >> 5) = ((e & 31u) == (e >> 5)) ? 1.0 : 0.0;
