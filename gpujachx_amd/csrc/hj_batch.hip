@@ -35,6 +35,10 @@ namespace {
 constexpr unsigned BLDR = 33u;                        // padded leading dimension
 constexpr unsigned WAVE_LDS = 2u * 32u * BLDR;        // doubles of R and W per wave
 constexpr unsigned BATCH_MAX_WAVES = 8u;
+// waves per SIMD the batch kernel is compiled for (register budget 512/n)
+#ifndef BATCH_WAVES_PER_SIMD
+#define BATCH_WAVES_PER_SIMD 2
+#endif
 
 struct BProb {
   double *G, *V, *D;
@@ -315,7 +319,7 @@ __global__ void k_batch_reset(unsigned *queue)
   if (threadIdx.x == 0u) *queue = 0u;
 }
 
-__global__ __launch_bounds__(512) void k_batch(const BProb *__restrict__ probs,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BATCH_WAVES_PER_SIMD))) void k_batch(const BProb *__restrict__ probs,
                                                const unsigned *__restrict__ order, unsigned nprob,
                                                unsigned *__restrict__ queue,
                                                const uint16_t *__restrict__ s1pool,

@@ -704,6 +704,17 @@ __device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
 #ifndef UPGRAM_PREFETCH
 #define UPGRAM_PREFETCH 1
 #endif
+// streaming (non-temporal) loads and stores of G in the fused kernel
+#ifndef UPGRAM_NT
+#define UPGRAM_NT 1
+#endif
+#if UPGRAM_NT
+#define UG_LOAD(p) __builtin_nontemporal_load(p)
+#define UG_STORE(v, p) __builtin_nontemporal_store(v, p)
+#else
+#define UG_LOAD(p) (*(p))
+#define UG_STORE(v, p) (*(p) = (v))
+#endif
 
 // Operands of one 16-row slab of a pair: for an updated pair the A operands
 // of [G_a G_b] W (lane l: row l%16 of pair column 4u + l/16), else the
@@ -717,7 +728,7 @@ __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned
     for (unsigned u = 0u; u < 8u; ++u) {
       const unsigned cc = 4u * u + (l >> 4);
       const unsigned gc = (cc < 16u) ? (16u * a + cc) : (16u * b + cc - 16u);
-      x[u] = G[(size_t)gc * ldG + row + (l & 15u)];
+      x[u] = UG_LOAD(G + (size_t)gc * ldG + row + (l & 15u));
     }
   }
   else {
@@ -725,8 +736,8 @@ __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned
     const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
 #pragma unroll
     for (unsigned r = 0u; r < 4u; ++r) {
-      x[r] = ia[4u * r];
-      x[4u + r] = ib[4u * r];
+      x[r] = UG_LOAD(ia + 4u * r);
+      x[4u + r] = UG_LOAD(ib + 4u * r);
     }
   }
 }
@@ -759,7 +770,7 @@ __device__ __forceinline__ void store_tile(double *__restrict__ G, unsigned ldG,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (unsigned q = 0u; q < 4u; ++q)
-    G[(size_t)(16u * blk + 4u * q + (l >> 4)) * ldG + row + (l & 15u)] = v[q];
+    UG_STORE(v[q], G + (size_t)(16u * blk + 4u * q + (l >> 4)) * ldG + row + (l & 15u));
 #else
   double *const o = G + (size_t)(16u * blk + (l & 15u)) * ldG + row + (l >> 4);
 #pragma unroll
