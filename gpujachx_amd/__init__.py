@@ -114,6 +114,12 @@ def lib():
     L.hj_mlp_predict.restype = i
     L.hj_mlp_train.argtypes = [pf, pf, u, u, u, ctypes.c_float, ctypes.c_float, u, pf, pf]
     L.hj_mlp_train.restype = i
+    L.hj_set_skip_min_frac.argtypes = [ctypes.c_float]
+    L.hj_set_skip_min_frac.restype = None
+    L.hj_set_skip_min_propose.argtypes = [ctypes.c_float]
+    L.hj_set_skip_min_propose.restype = None
+    L.hj_set_skip_min_visits.argtypes = [ctypes.c_uint]
+    L.hj_set_skip_min_visits.restype = None
     L.hj_set_split_tau.argtypes = [ctypes.c_float]
     L.hj_set_split_tau.restype = None
     L.hj_last_splits.argtypes = [pu, pull]

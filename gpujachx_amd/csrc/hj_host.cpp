@@ -41,6 +41,15 @@ static uint16_t g_model_bf16[MLP_NPARAM];
 static float g_skip_threshold = 0.05f;
 static unsigned g_skip_max = 2u;
 static float g_split_tau = 1e-3f;  // split Gram while the pair's last max |cos| > tau
+// skipping only while the previous sweep's big rotations are at least this
+// fraction of the most a sweep can have (the last sweeps run unskipped)
+static float g_skip_min_frac = 0.01f;
+// ... and only in sweeps where the model proposes at least this fraction of
+// all visits
+static float g_skip_min_propose = 0.1f;
+// ... and only in runs with at least this many visits per sweep (small runs
+// converge in few sweeps; one sweep added costs more than skipping saves)
+static unsigned g_skip_min_visits = 32000u;
 // per-sweep visit records copied to the host (trace collection)
 static void *g_trace_buf = nullptr;
 static unsigned g_trace_sweeps = 0u;
@@ -314,7 +323,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     launch_initV(s, dV, 0u, ncol, lddV);
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
   double *const V = full_svd ? dV : nullptr;
-  const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready;
+  const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready &&
+                       (size_t)(ncol / 32u) * STRAT1_STEPS >= g_skip_min_visits;
   const bool split = (g_form & HJ_FORM_SPLIT_GRAM) != 0u;
   const bool planned = learned || split;
   if (learned)
@@ -325,6 +335,20 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   g_stats = Stats();
   *glb_s = 0ull;
   *glb_b = 0ull;
+  {
+    // the forms' policy, once per run (k_sweep_begin applies it every sweep)
+    RunCtl &c = *w.h_ctl;
+    c.learned = learned ? 1u : 0u;
+    c.split_form = split ? 1u : 0u;
+    c.threshold = g_skip_threshold;
+    c.max_skips = g_skip_max;
+    c.tau = g_split_tau;
+    c.min_frac = g_skip_min_frac;
+    c.min_skip_frac = g_skip_min_propose;
+    c.max_rot = max_rot;
+    HIP_CALL(hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(RunCtl), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipStreamSynchronize(s));
+  }
   const bool eager = g_profile || (g_step_limit > 0) || getenv("HJ_NOGRAPH");
   hipGraphExec_t gexec = nullptr;
   std::vector<hipEvent_t> evs;
@@ -356,28 +380,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   stopwatch_reset(swp_tim);
   unsigned blk_swp = 0u;
   long steps_done = 0;
-  bool stop = false, certify = false;
+  bool stop = false;
   unsigned issued = 0u;
   std::vector<char> gram_launched(w.steps, 0);
   g_stats.skipped = 0ull;
   while (blk_swp < g_max_sweeps) {
-    if (planned) {
-      // skipping on unless this is the first or a certification sweep; the
-      // split Gram on unless this is a certification sweep
-      RunCtl &c = *w.h_ctl;
-      c.skip_on = (learned && blk_swp > 0u && !certify) ? 1u : 0u;
-      c.threshold = g_skip_threshold;
-      c.max_skips = g_skip_max;
-      c.split_on = (split && !certify) ? 1u : 0u;
-      c.tau = g_split_tau;
-      HIP_CALL(hipMemcpyAsync(reinterpret_cast<char *>(w.d_ctl) + offsetof(RunCtl, skip_on), &c.skip_on,
-                              offsetof(RunCtl, skipped) - offsetof(RunCtl, skip_on),
-                              hipMemcpyHostToDevice, s));
-      HIP_CALL(hipMemcpyAsync(reinterpret_cast<char *>(w.d_ctl) + offsetof(RunCtl, split_on), &c.split_on,
-                              offsetof(RunCtl, split) - offsetof(RunCtl, split_on),
-                              hipMemcpyHostToDevice, s));
-      HIP_CALL(hipStreamSynchronize(s));
-    }
     if (eager) {
       sweep_head(s);
       long left = (g_step_limit > 0) ? (g_step_limit - steps_done) : (long)w.steps;
@@ -431,14 +438,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     }
     if (stop)
       break;
-    if (!cvg_b) {
-      if (skipped || nsplit) {
-        certify = true;  // the next sweep runs every visit, all in FP64
-        continue;
-      }
+    // no big rotation ends the run only in an all-FP64, unskipped sweep;
+    // otherwise the next sweep certifies (k_sweep_begin switches the forms
+    // off for it)
+    if (!cvg_b && !skipped && !nsplit)
       break;
-    }
-    certify = false;
   }
   if (gexec) HIP_CALL(hipGraphExecDestroy(gexec));
   for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
@@ -630,6 +634,12 @@ extern "C" void hj_last_skips(unsigned *skip, unsigned long long *skipped, unsig
 }
 
 extern "C" void hj_set_split_tau(float tau) { g_split_tau = tau; }
+
+extern "C" void hj_set_skip_min_frac(float frac) { g_skip_min_frac = frac; }
+
+extern "C" void hj_set_skip_min_propose(float frac) { g_skip_min_propose = frac; }
+
+extern "C" void hj_set_skip_min_visits(unsigned visits) { g_skip_min_visits = visits; }
 
 extern "C" void hj_last_splits(unsigned *split, unsigned long long *total)
 {

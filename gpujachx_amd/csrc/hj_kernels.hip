@@ -974,14 +974,27 @@ __global__ __launch_bounds__(256) void k_initV(double *__restrict__ V, unsigned 
 
 // Start of a level-2 sweep: keep the previous counter, clear the running
 // one, advance the sweep index (a kernel, so no memset node in the graph).
+// Start of a level-2 sweep: the previous sweep's counter becomes prev_cvg,
+// the counters restart, and the forms' switches for this sweep follow from
+// the previous sweep: a sweep after one that had skipped or split visits and
+// no big rotation certifies (every visit, FP64); otherwise skipping is on
+// when the form is and the previous sweep's big-rotation fraction reached
+// min_frac, the split Gram when the form is.
 __global__ void k_sweep_begin(RunCtl *ctl)
 {
   if (threadIdx.x == 0u) {
+    const unsigned prev_b = (unsigned)(ctl->cvg >> 32);
+    const bool first = (ctl->sweep == 0u);
+    const bool certify = !first && !prev_b && (ctl->skipped || ctl->split);
+    const float frac = (float)prev_b / ctl->max_rot;
+    ctl->skip_on = (ctl->learned && !first && !certify && frac >= ctl->min_frac) ? 1u : 0u;
+    ctl->split_on = (ctl->split_form && !certify) ? 1u : 0u;
     ctl->prev_cvg = ctl->cvg;
     ctl->cvg = 0ull;
     ctl->sweep += 1u;
     ctl->skipped = 0u;
     ctl->split = 0u;
+    ctl->proposed = 0u;
   }
 }
 

@@ -33,6 +33,9 @@ constexpr unsigned REC_SPLIT_FAIL = 2u;  // ... whose Cholesky failed: no update
 constexpr uint8_t PLAN_FP64 = 0u;   // FP64 Gram
 constexpr uint8_t PLAN_SKIP = 1u;   // skipped
 constexpr uint8_t PLAN_SPLIT = 2u;  // split-precision Gram
+// proposals of k_plan, settled by k_plan_gate before the steps run
+constexpr uint8_t PLAN_PROP_SKIP_FP64 = 3u;   // skip, else FP64
+constexpr uint8_t PLAN_PROP_SKIP_SPLIT = 4u;  // skip, else split
 
 // Device-side control words of one level-2 run.
 struct RunCtl {
@@ -46,6 +49,14 @@ struct RunCtl {
   unsigned split_on;            // split-precision Gram allowed in this sweep
   float tau;                    // ... for pairs whose last max |cos| exceeds tau
   unsigned split;               // visits with the split Gram in the running sweep
+  // policy of the forms, set once per run; k_sweep_begin derives skip_on /
+  // split_on of every sweep from it and the previous sweep's counters
+  unsigned learned;             // skipping form on
+  unsigned split_form;          // split form on
+  float min_frac;               // skip only after a sweep with at least this big-rotation fraction
+  float min_skip_frac;          // ... and only if the model proposes at least this fraction of skips
+  float max_rot;                // most big rotations a sweep can have (visits x 496)
+  unsigned proposed;            // skips the model proposed for the running sweep
 };
 
 // waves sharing one Gram chunk of CH rows: each takes CH/nw rows, a multiple

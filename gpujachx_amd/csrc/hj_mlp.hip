@@ -106,7 +106,7 @@ __device__ float mlp_logit(const float *w, const float *f)
 __global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, unsigned nkeys,
                                               const uint16_t *__restrict__ strat1, unsigned nplus,
                                               float log2n1, float max_rot,
-                                              const RunCtl *__restrict__ ctl,
+                                              RunCtl *__restrict__ ctl,
                                               const uint16_t *__restrict__ model, int learned,
                                               uint8_t *__restrict__ plan)
 {
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, 
   if (key >= nkeys)
     return;
   const VisitRec v = rec[key];
-  uint8_t s = PLAN_FP64;
+  bool skip = false;
   if (learned && ctl->skip_on && v.sweep != 0u && v.skips < ctl->max_skips) {
     const float prev = (float)(unsigned)(ctl->prev_cvg >> 32) / max_rot;
     float f[MLP_F];
@@ -128,13 +128,34 @@ __global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, 
                    log2n1, f);
     const float z = mlp_logit(w, f);
     const float p = 1.0f / (1.0f + expf(-z));
-    if (p < ctl->threshold)
-      s = PLAN_SKIP;
+    skip = (p < ctl->threshold);
   }
-  if (s == PLAN_FP64 && ctl->split_on &&
-      (v.sweep == 0u || (v.maxcos > ctl->tau && !(v.flags & REC_SPLIT_FAIL))))
-    s = PLAN_SPLIT;
+  const bool split = ctl->split_on &&
+                     (v.sweep == 0u || (v.maxcos > ctl->tau && !(v.flags & REC_SPLIT_FAIL)));
+  uint8_t s = split ? PLAN_SPLIT : PLAN_FP64;
+  if (skip)
+    s = split ? PLAN_PROP_SKIP_SPLIT : PLAN_PROP_SKIP_FP64;
+  // proposals counted per wave
+  const unsigned long long b = __ballot(skip);
+  if ((threadIdx.x & 63u) == 0u && b)
+    atomicAdd(&ctl->proposed, (unsigned)__popcll(b));
   plan[key] = s;
+}
+
+// Settles the proposals: the sweep skips the proposed visits only if they
+// are at least min_skip_frac of all (a few skips cannot pay for the sweeps
+// they may add); otherwise they run, FP64 or split as planned.
+__global__ __launch_bounds__(256) void k_plan_gate(unsigned nkeys, const RunCtl *__restrict__ ctl,
+                                                   uint8_t *__restrict__ plan)
+{
+  const unsigned key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= nkeys)
+    return;
+  const uint8_t s = plan[key];
+  if (s < PLAN_PROP_SKIP_FP64)
+    return;
+  const bool go = (float)ctl->proposed >= ctl->min_skip_frac * (float)nkeys;
+  plan[key] = go ? PLAN_SKIP : ((s == PLAN_PROP_SKIP_SPLIT) ? PLAN_SPLIT : PLAN_FP64);
 }
 
 // Features of many records (training-set construction).
@@ -246,11 +267,14 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__re
 }
 
 void launch_plan(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
-                 unsigned nplus, float log2n1, float max_rot, const RunCtl *ctl,
+                 unsigned nplus, float log2n1, float max_rot, RunCtl *ctl,
                  const uint16_t *model, bool learned, uint8_t *plan)
 {
   hipLaunchKernelGGL(k_plan, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, rec, nkeys, strat1, nplus,
                      log2n1, max_rot, ctl, model, learned ? 1 : 0, plan);
+  if (learned)
+    hipLaunchKernelGGL(k_plan_gate, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, nkeys,
+                       (const RunCtl *)ctl, plan);
 }
 
 // Device buffers of one training / gradient evaluation.

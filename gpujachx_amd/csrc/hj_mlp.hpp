@@ -21,7 +21,7 @@ constexpr unsigned MLP_OFF_B3 = MLP_OFF_W3 + MLP_H;
 constexpr unsigned MLP_NPARAM = MLP_OFF_B3 + 1u;
 
 void launch_plan(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
-                 unsigned nplus, float log2n1, float max_rot, const RunCtl *ctl,
+                 unsigned nplus, float log2n1, float max_rot, RunCtl *ctl,
                  const uint16_t *model, bool learned, uint8_t *plan);
 
 }  // namespace hj

@@ -154,13 +154,27 @@ def enable(path=DEFAULT_MODEL, threshold=None, max_skips=None, form=None):
 
 
 def _configs():
-    """Training inputs: the plain method's traces on the first three
-    benchmark configurations (SVD 256, HSVD 4096x2048, SVD 8192)."""
+    """Training inputs: the plain method's traces on SVD 256, HSVD
+    4096x2048, SVD 2048, SVD 8192 and HSVD 8192 (NPLUS = N/2), with seeds
+    other than the benchmarks'."""
     return [
         ("svd 256", 256, 256, 256, 101),
         ("hsvd 4096x2048", 4096, 2048, 1024, 102),
+        ("svd 2048", 2048, 2048, 2048, 105),
         ("svd 8192", 8192, 8192, 8192, 103),
+        ("hsvd 8192", 8192, 8192, 4096, 104),
     ]
+
+
+def balance(Xs, ys, per_config, rng):
+    """Equal weight per configuration: each trace contributes per_config
+    samples (subsampled, or repeated when it has fewer)."""
+    Xb, yb = [], []
+    for X, y in zip(Xs, ys):
+        idx = rng.choice(X.shape[0], per_config, replace=X.shape[0] < per_config)
+        Xb.append(X[idx])
+        yb.append(y[idx])
+    return np.concatenate(Xb), np.concatenate(yb)
 
 
 def _make_input(torch, m, n, nplus, seed, kind):
@@ -214,7 +228,7 @@ def evaluate(path, seeds=(201,), sizes=None, thresholds=(None,), max_skips=(None
             Gt = torch.from_numpy(_make_input(torch, m, n, nplus, seed,
                                               "hsvd" if nplus < n else "svd").T.copy()).cuda()
             D = torch.empty(n, dtype=torch.float64, device="cuda")
-            alg = 11 if nplus < n else 9
+            alg = 11 if (nplus < n and m != n) else 9
             g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
             old = g.set_max_sweeps(max_sweeps)
             try:
@@ -272,20 +286,17 @@ def main():
         if n > args.max_n:
             continue
         t0 = time.time()
-        G = _make_input(torch, m, n, nplus, seed, "hsvd" if nplus < n else "svd")
+        G = _make_input(torch, m, n, nplus, seed, "hsvd" if (nplus < n and m != n) else "svd")
         g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
-        tr, s1, rb, r = collect_trace(11 if nplus < n else 9, G, nplus)
+        tr, s1, rb, r = collect_trace(11 if (nplus < n and m != n) else 9, G, nplus)
         X, y = dataset(tr, s1, rb, nplus, n // 16)
         meta["configs"].append({"name": name, "sweeps": int(r.sweeps), "samples": int(X.shape[0]),
                                 "positives": float(y.mean()), "seconds": time.time() - t0})
         print(name, meta["configs"][-1], flush=True)
         Xs.append(X)
         ys.append(y)
-    X, y = np.concatenate(Xs), np.concatenate(ys)
     rng = np.random.default_rng(0)
-    if X.shape[0] > args.max_samples:
-        sel = rng.choice(X.shape[0], args.max_samples, replace=False)
-        X, y = X[sel], y[sel]
+    X, y = balance(Xs, ys, args.max_samples // len(Xs), rng)
     perm = rng.permutation(X.shape[0])
     nval = max(1, X.shape[0] // 10)
     val, trn = perm[:nval], perm[nval:]

@@ -134,6 +134,16 @@ void hj_set_form(unsigned form);
  * Returns -1 on a size mismatch. */
 int hj_set_skip_model(const float *params, unsigned nparam, float threshold,
                       unsigned max_skips);
+/* Skipping only in sweeps whose previous sweep had at least this fraction
+ * of the possible big rotations (default 0.01): the final sweeps of a run go
+ * unskipped.  And only in sweeps where the model proposes at least
+ * min_propose of all visits (default 0.1): a few skips cannot pay for the
+ * sweeps they may add. */
+void hj_set_skip_min_frac(float frac);
+void hj_set_skip_min_propose(float min_propose);
+/* Skipping only in runs with at least this many block-pair visits per sweep
+ * (default 32000, i.e. N >= 4096 with 32-column pairs). */
+void hj_set_skip_min_visits(unsigned visits);
 unsigned hj_mlp_nparam(void);
 unsigned hj_mlp_nfeat(void);
 unsigned hj_visit_record_size(void);

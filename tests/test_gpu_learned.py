@@ -17,6 +17,7 @@ def _device():
     g.configureGPU(0)
     yield
     g.set_form(g.FORM_PLAIN)
+    g.lib().hj_set_skip_min_visits(32000)
 
 
 def _data(n, seed):
@@ -80,10 +81,14 @@ def _constant_model(logit):
     return p
 
 
-def _use_model(p, thr=0.05, max_skips=2):
+def _use_model(p, thr=0.05, max_skips=2, min_frac=0.0, min_propose=0.0):
     import ctypes
     assert g.lib().hj_set_skip_model(p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size,
                                      thr, max_skips) == 0
+    # skipping allowed in every sweep and run (the policy is tested apart)
+    g.lib().hj_set_skip_min_visits(0)
+    g.lib().hj_set_skip_min_frac(min_frac)
+    g.lib().hj_set_skip_min_propose(min_propose)
     g.set_form(g.FORM_LEARNED_SKIP)
 
 
@@ -102,6 +107,8 @@ def test_skip_none_is_plain(alg, kind):
     _use_model(_constant_model(+30.0))
     rl = g.hypJacL2(alg, G, nplus)
     g.set_form(g.FORM_PLAIN)
+    g.lib().hj_set_skip_min_frac(0.01)
+    g.lib().hj_set_skip_min_propose(0.1)
     assert rl.sweeps == rp.sweeps and rl.rot_b == rp.rot_b
     np.testing.assert_array_equal(rl.G, rp.G)
     np.testing.assert_array_equal(rl.D, rp.D)
@@ -128,6 +135,8 @@ def test_skip_all_is_plain_with_idle_sweeps(alg, kind):
     finally:
         g.set_form(g.FORM_PLAIN)
         g.set_max_sweeps(old)
+        g.lib().hj_set_skip_min_frac(0.01)
+        g.lib().hj_set_skip_min_propose(0.1)
     np.testing.assert_array_equal(rl.G, rp.G)
     np.testing.assert_array_equal(rl.D, rp.D)
     if rl.V is not None:
@@ -146,6 +155,7 @@ def test_learned_model_file_converges():
     G = svd_input(1024, 1024, 10.0 ** (-10.0 * np.random.default_rng(3).uniform(size=1024)), seed=31)
     rp = _plain(9, G, 1024)
     lr.enable()
+    g.lib().hj_set_skip_min_visits(0)
     try:
         rl = g.hypJacL2(9, G, 1024)
         sk, tot, vis = g.last_skips()
@@ -153,5 +163,76 @@ def test_learned_model_file_converges():
     finally:
         g.set_form(g.FORM_PLAIN)
     assert rb[-1] == 0 and sk[-1] == 0
+    a, b = np.sort(rp.sigma), np.sort(rl.sigma)
+    assert np.abs(a - b).max() <= 1e-13 * a.max()
+
+
+def test_skip_policy_gates():
+    # with min_propose above 1 no sweep may skip: the run is the plain run;
+    # with min_frac above 1 neither
+    G = graded_input(512, 256, seed=23)
+    rp = _plain(9, G, 256)
+    for mf, mp in ((0.0, 1.5), (1.5, 0.0)):
+        _use_model(_constant_model(-30.0), min_frac=mf, min_propose=mp)
+        try:
+            rl = g.hypJacL2(9, G, 256)
+            _, tot, _ = g.last_skips()
+        finally:
+            g.set_form(g.FORM_PLAIN)
+            g.lib().hj_set_skip_min_frac(0.01)
+            g.lib().hj_set_skip_min_propose(0.1)
+        assert tot == 0
+        np.testing.assert_array_equal(rl.G, rp.G)
+        assert rl.sweeps == rp.sweeps
+
+
+def test_small_runs_do_not_skip():
+    # below the visits-per-sweep floor the learned form runs the plain method
+    G = graded_input(512, 256, seed=24)
+    rp = _plain(9, G, 256)
+    _use_model(_constant_model(-30.0))
+    g.lib().hj_set_skip_min_visits(32000)
+    try:
+        rl = g.hypJacL2(9, G, 256)
+        _, tot, _ = g.last_skips()
+    finally:
+        g.set_form(g.FORM_PLAIN)
+        g.lib().hj_set_skip_min_frac(0.01)
+        g.lib().hj_set_skip_min_propose(0.1)
+    assert tot == 0 and rl.sweeps == rp.sweeps
+    np.testing.assert_array_equal(rl.G, rp.G)
+
+
+def test_learned_gain_at_4096():
+    # what the trained network gains: at N = 4096 (above the visits floor)
+    # the packaged model executes clearly fewer block-pair visits than the
+    # plain method, ends on an unskipped converged sweep, and gives the same
+    # sigma; both wall times are printed
+    import os
+    import time
+    if not os.path.exists(lr.DEFAULT_MODEL):
+        pytest.skip("no trained model file")
+    n = 4096
+    s = 10.0 ** (-12.0 * np.random.default_rng(8).uniform(size=n))
+    G = svd_input(n, n, s, seed=33)
+    old = g.set_max_sweeps(80)
+    try:
+        t0 = time.perf_counter()
+        rp = _plain(9, G, n)
+        tp = time.perf_counter() - t0
+        _, _, vp = g.last_skips()
+        lr.enable()
+        t0 = time.perf_counter()
+        rl = g.hypJacL2(9, G, n)
+        tl = time.perf_counter() - t0
+        sk, tot, vl = g.last_skips()
+        _, rb, _ = g.last_sweeps()
+    finally:
+        g.set_form(g.FORM_PLAIN)
+        g.set_max_sweeps(old)
+    print("N=4096: plain %.3f s %d sweeps %d visits; learned %.3f s %d sweeps %d visits (%d skipped)"
+          % (tp, rp.sweeps, vp, tl, rl.sweeps, vl, tot))
+    assert rb[-1] == 0 and sk[-1] == 0
+    assert vl <= 0.9 * vp, (vl, vp)
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()

@@ -110,6 +110,7 @@ def test_learned_plus_split_matches_plain():
         rp, _, _, rbp = _run(g.FORM_PLAIN, 9, G, 1024)
         g.init_strats_builtin("cycwor", 32, "cycwor", 64)
         lr.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
+        g.lib().hj_set_skip_min_visits(0)
         rl = g.hypJacL2(9, G, 1024)
         sp, _ = g.last_splits()
         sk, _, _ = g.last_skips()
@@ -117,6 +118,7 @@ def test_learned_plus_split_matches_plain():
     finally:
         g.set_form(g.FORM_PLAIN)
         g.set_max_sweeps(old)
+        g.lib().hj_set_skip_min_visits(32000)
     assert rbp[-1] == 0
     assert rb[-1] == 0 and sp[-1] == 0 and sk[-1] == 0
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
