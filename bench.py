@@ -221,13 +221,22 @@ def main():
     upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     gram_tf = pairs_per_launch * 1536.0 * n / (gram_ms * 1e-3) / 1e12
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "update_traffic.json")
+    # HBM traffic per launch: FETCH_SIZE + WRITE_SIZE bytes per executed pair
+    # visit of k_upgram, from counter runs of lib/hj_pmc on full-step launches
+    # at N=8192 (each counter scaled by its reading for a copy of known size;
+    # gpujachx_amd/pmc_traffic.py), times this run's executed visits per
+    # launch, scaled linearly in the row count
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, "gpujachx_amd", "data", "upgram_traffic.json")
     if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        tj = json.load(open(tfile))
+        up = tj["kernels"]["k_upgram"]
+        traffic = up["bytes_per_visit"] * pairs_per_launch * n / tj["n"]
+        traffic_src = {"file": "gpujachx_amd/data/upgram_traffic.json",
+                       "bytes_per_visit_at_n": [up["bytes_per_visit"], tj["n"]],
+                       "traffic_over_algorithmic": up["traffic_over_algorithmic"],
+                       "counter_scale": [tj["calibration"]["fetch_scale"],
+                                         tj["calibration"]["write_scale"]]}
 
     out = {
         "metric": METRIC,
@@ -259,7 +268,7 @@ def main():
                       "normwise": float(normwise),
                       "frac_rel_le_1e-12": float(np.mean(rel <= 1e-12))},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_upgram (block update + next-step Gram)", "avg_launch_ms": upd_ms,
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},

@@ -115,6 +115,14 @@ def build_tools():
         _run(["g++"] + common + ["-o", os.path.join(LIB, name)] + paths + extra + link_h5)
 
 
+def build_pmc():
+    """hj_pmc: the profiling driver without Python (counter runs)."""
+    out = os.path.join(LIB, "hj_pmc")
+    _run([HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-o", out,
+          os.path.join(CSRC, "pmc_main.cpp"), "-L" + LIB, "-Wl,-rpath,$ORIGIN", "-lhypjac"])
+    return out
+
+
 def build_oracle(ref=True):
     out = os.path.join(ORACLE, "libhjoracle.so")
     _run(["gcc", "-O2", "-mfma", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC",
@@ -129,6 +137,7 @@ def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     build_oracle(ref="--no-ref" not in argv)
     build_library()
+    build_pmc()
     build_strat()
     build_tools()
 

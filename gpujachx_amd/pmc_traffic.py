@@ -1,0 +1,77 @@
+"""HBM bytes per launch of the fused update kernel from two counter runs of
+the profiling driver `lib/hj_pmc` (one FETCH_SIZE pass, one WRITE_SIZE pass,
+rocprofv3 CSV output).  Each counter is scaled by what it reads for the
+driver's calibration copy (a device-to-device copy of known size: its reads
+and its writes are each COPY bytes), so a counter that under-counts by a
+fixed factor on this GPU is corrected by that factor.
+
+    cd /tmp && export TMPDIR=/tmp
+    timeout -s KILL 90 rocprofv3 --pmc FETCH_SIZE --output-format csv -d /tmp/pmc_FETCH_SIZE -o pmc \
+        -- <repo>/gpujachx_amd/lib/hj_pmc 8192 8
+    (the same with WRITE_SIZE)
+    python -m gpujachx_amd.pmc_traffic FETCH.csv WRITE.csv [--copy-mib 1024] [--out FILE]
+
+Writes a JSON file with bytes per launch and per executed pair visit
+(N x 32 columns read and written) for k_upgram, k_update and k_gram.
+"""
+import argparse
+import csv
+import json
+import os
+
+KERNELS = {"k_upgram": "hj::k_upgram(", "k_update": "hj::k_update(", "k_gram": "hj::k_gram(",
+           "k_pair": "hj::k_pair"}
+
+
+def per_kernel(path):
+    rows = list(csv.DictReader(open(path)))
+    out = {"copy": None}
+    for r in rows:
+        name, v = r["Kernel_Name"], float(r["Counter_Value"])
+        if name.startswith("__amd_rocclr_copyBuffer"):
+            # the calibration copy is the largest copy dispatch
+            out["copy"] = v if out["copy"] is None else max(out["copy"], v)
+            continue
+        for k, pre in KERNELS.items():
+            if pre in name:
+                out.setdefault(k, []).append(v)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("fetch_csv")
+    ap.add_argument("write_csv")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                                  "upgram_traffic.json"))
+    args = ap.parse_args(argv)
+    f, w = per_kernel(args.fetch_csv), per_kernel(args.write_csv)
+    copy_kib = args.copy_mib * 1024.0
+    fs, ws = copy_kib / f["copy"], copy_kib / w["copy"]
+    visits = args.n // 32            # pairs per step when none is skipped
+    res = {"n": args.n, "visits_per_launch": visits,
+           "calibration": {"copy_mib": args.copy_mib, "fetch_kib": f["copy"], "write_kib": w["copy"],
+                           "fetch_scale": fs, "write_scale": ws},
+           "kernels": {}}
+    for k in KERNELS:
+        if k not in f or k not in w:
+            continue
+        fk = sum(f[k]) / len(f[k])
+        wk = sum(w[k]) / len(w[k])
+        b = (fk * fs + wk * ws) * 1024.0
+        alg = {"k_upgram": 512.0, "k_update": 512.0, "k_gram": 256.0}.get(k)
+        ent = {"dispatches": len(f[k]), "fetch_kib": fk, "write_kib": wk,
+               "bytes_per_launch": b, "bytes_per_visit": b / visits}
+        if alg is not None:
+            ent["algorithmic_bytes_per_launch"] = alg * args.n * visits
+            ent["traffic_over_algorithmic"] = b / (alg * args.n * visits)
+        res["kernels"][k] = ent
+    with open(args.out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
