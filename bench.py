@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-3,
                     help="split Gram while a pair's last max |cos| exceeds tau")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--flow", choices=["on", "off"], default="off",
+                    help="dataflow sweeps (one persistent launch per sweep) for both forms")
     ap.add_argument("--model", default=None, help="skip-model file (default: the packaged one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dgesvj", action="store_true", help="also time LAPACK dgesvj at N=1024")
@@ -126,6 +128,7 @@ def main():
     g.configureGPU(0)
     g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
     g.set_max_sweeps(args.max_sweeps)
+    g.set_dataflow(args.flow == "on")
     L = g.lib()
     G0, s_exact = make_input(torch, n, args.seed)
     W = torch.empty_like(G0)
@@ -164,13 +167,13 @@ def main():
         for _ in range(warmup):
             one()
         g.set_profile(profile)
-        kms, kcnt = np.zeros(3), np.zeros(3)
+        kms, kcnt = np.zeros(4), np.zeros(4)
         visits = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             one()
-            for k in range(3):
+            for k in range(4):
                 ms, c = g.kernel_ms(k)
                 kms[k] += ms
                 kcnt[k] += c
@@ -183,6 +186,7 @@ def main():
         splits = g.last_splits()
         sig = torch.sqrt(torch.abs(D)).cpu().numpy()
         return {"per": elapsed / steps, "kms": kms, "kcnt": kcnt, "visits": visits,
+                "flow": g.last_dataflow(),
                 "sweeps": sw.value, "rot_b": rot_b, "skipped": skips[1], "split": splits[1],
                 "split_per_sweep": splits[0], "skip_per_sweep": skips[0], "sigma": sig}
 
@@ -226,6 +230,17 @@ def main():
     # at N=8192 (each counter scaled by its reading for a copy of known size;
     # gpujachx_amd/pmc_traffic.py), times this run's executed visits per
     # launch, scaled linearly in the row count
+    flow_used = bool(main_run["flow"])
+    if flow_used:
+        # the dataflow launch holds the fused transitions of steps 0..S-2 of
+        # a sweep (the last step's update is a k_update after it) and every
+        # pair visit; its roofline is the HBM stream of those transitions
+        fl_n = max(kcnt[3], 1)
+        pairs_per_launch = (main_run["visits"] / fl_n) * (steps_per_sweep - 1) / steps_per_sweep
+        upd_bytes = pairs_per_launch * 512.0 * n
+        upd_ms = kms[3] / fl_n
+        ach = upd_bytes / (upd_ms * 1e-3) / 1e9
+        upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "gpujachx_amd", "data", "upgram_traffic.json")
     if os.path.exists(tfile):
@@ -269,13 +284,16 @@ def main():
                       "frac_rel_le_1e-12": float(np.mean(rel <= 1e-12))},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_upgram (block update + next-step Gram)", "avg_launch_ms": upd_ms,
+                     "kernel": ("k_flow (dataflow sweep: fused block updates + Grams, pair visits "
+                                "overlapped)") if flow_used else "k_upgram (block update + next-step Gram)",
+                     "avg_launch_ms": upd_ms,
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},
         "kernels": {"gram": {"avg_ms": gram_ms, "GBs": gram_ach, "tflops": gram_tf},
                     "pair": {"avg_ms": pair_ms},
                     "update": {"avg_ms": upd_ms, "GBs": ach, "tflops": upd_tf},
-                    "launches_per_step": float(kcnt[2] / args.steps)},
+                    "launches_per_step": float((kcnt[3] if flow_used else kcnt[2]) / args.steps)},
+        "dataflow": flow_used,
     }
     if cmp_run is not None:
         cs = np.sort(cmp_run["sigma"])[::-1]

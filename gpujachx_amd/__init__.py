@@ -90,6 +90,10 @@ def lib():
     L.hj_set_max_sweeps.restype = u
     L.hj_set_step_limit.argtypes = [ctypes.c_long]
     L.hj_set_step_limit.restype = None
+    L.hj_set_dataflow.argtypes = [i]
+    L.hj_set_dataflow.restype = i
+    L.hj_last_dataflow.argtypes = []
+    L.hj_last_dataflow.restype = i
     L.hj_dbg_scalar.argtypes = [i, i, pd, pd, pd, pd, pd, ctypes.POINTER(ctypes.c_int)]
     L.hj_dbg_scalar.restype = i
     L.hj_version.argtypes = []
@@ -257,6 +261,17 @@ def set_step_limit(steps):
 
 def set_form(form):
     lib().hj_set_form(int(form))
+
+
+def set_dataflow(on):
+    """Dataflow sweeps on/off (one persistent launch per sweep where the
+    strategy and routine allow it); returns the previous setting."""
+    return bool(lib().hj_set_dataflow(1 if on else 0))
+
+
+def last_dataflow():
+    """Whether the last hypJacL2 / hypJacL2_dev call ran dataflow sweeps."""
+    return bool(lib().hj_last_dataflow())
 
 
 def last_skips():

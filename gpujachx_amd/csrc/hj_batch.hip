@@ -32,7 +32,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 namespace hj {
 namespace {
 
-constexpr unsigned BLDR = 33u;                        // padded leading dimension
+constexpr unsigned BLDR = L0LD;                       // padded leading dimension (the level-0 layout)
 constexpr unsigned WAVE_LDS = 2u * 32u * BLDR;        // doubles of R and W per wave
 constexpr unsigned BATCH_MAX_WAVES = 8u;
 // waves per SIMD the batch kernel is compiled for (register budget 512/n)
@@ -60,26 +60,6 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
 }
 
 #define BR(A, r, c) (A)[(c) * BLDR + (r)]
-
-// Sum of the 32 rows of one pivot pair held by 4 lanes, lane r holding rows
-// r + 4k as v[k]: the xor butterfly 16,8,4,2,1 of the reference's 32-lane
-// reduction (levels 16, 8 and 4 inside the lane, 2 and 1 across lanes).
-__device__ __forceinline__ double sum4x8(const double *v)
-{
-  const double a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
-  double s = (a0 + a2) + (a1 + a3);
-  s = s + lane_xor2(s);
-  s = s + lane_xor1(s);
-  return s;
-}
-
-__device__ __forceinline__ unsigned wave_sum_u(unsigned v)
-{
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1)
-    v += (unsigned)__shfl_xor((int)v, o, 64);
-  return v;
-}
 
 // Gram of the pair (columns 16 iBlk.., 16 jBlk..) over all nrow rows by one
 // wave: 16-row slabs in order, MFMA step j taking row 4j + k in k-slot k
@@ -222,89 +202,11 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  // level-0 sweeps: slot y = l/4 (16 pivot pairs), lane r = l%4 of the slot
-  // holds rows r + 4k of the pair's columns of R and W
-  const unsigned y = l >> 2, r = l & 3u;
+  // level-0 sweeps (slot y = l/4 of the 16 pivot pairs, lane l%4 of the
+  // slot holding rows l%4 + 4k)
   unsigned blk_s = 0u, blk_b = 0u;
-  for (unsigned swp = 0u; swp < nswp; ++swp) {
-    unsigned my_s = 0u, my_b = 0u;
-    for (unsigned step = 0u; step < s0steps; ++step) {
-      const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
-      double Gp[8], Gq[8], Vp[8], Vq[8], t0[8], t1[8], t2[8];
-#pragma unroll
-      for (unsigned k = 0u; k < 8u; ++k) {
-        Gp[k] = BR(R, r + 4u * k, p);
-        Gq[k] = BR(R, r + 4u * k, q);
-        Vp[k] = BR(W, r + 4u * k, p);
-        Vq[k] = BR(W, r + 4u * k, q);
-      }
-#pragma unroll
-      for (unsigned k = 0u; k < 8u; ++k) {
-        t0[k] = __fma_rn(Gp[k], Gp[k], 0.0);
-        t1[k] = __fma_rn(Gq[k], Gq[k], 0.0);
-        t2[k] = __fma_rn(Gp[k], Gq[k], 0.0);
-      }
-      double Dp = sum4x8(t0), Dq = sum4x8(t1);
-      const double Apq = sum4x8(t2);
-      const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
-      const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
-      int pdef = def;
-      if (hyp) {
-        if ((p < npos) && (q < npos)) pdef = +1;
-        else if ((p >= npos) && (q >= npos)) pdef = -1;
-        else pdef = 0;
-      }
-      int transf_b = 0;
-      __builtin_amdgcn_wave_barrier();
-      if (transf_s) {
-        double c, t, t_;
-        bool trig;
-        if (!SORT) trig = !hyp || pdef;
-        else trig = (pdef != 0);
-        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
-        t_ = trig ? -t : t;
-        if (SORT && trig) {
-          Dp = __fma_rn(t, Apq, Dp);
-          Dq = __fma_rn(t_, Apq, Dq);
-        }
-        bool keep = true;
-        if (SORT)
-          keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
-        const unsigned PP = keep ? p : q, QQ = keep ? q : p;
-#pragma unroll
-        for (unsigned k = 0u; k < 8u; ++k) {
-          const double rp = __fma_rn(t_, Gq[k], Gp[k]), rq = __fma_rn(t, Gp[k], Gq[k]);
-          const double wp = __fma_rn(t_, Vq[k], Vp[k]), wq = __fma_rn(t, Vp[k], Vq[k]);
-          BR(R, r + 4u * k, PP) = transf_b ? c * rp : rp;
-          BR(R, r + 4u * k, QQ) = transf_b ? c * rq : rq;
-          BR(W, r + 4u * k, PP) = transf_b ? c * wp : wp;
-          BR(W, r + 4u * k, QQ) = transf_b ? c * wq : wq;
-        }
-      }
-      else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
-#pragma unroll
-        for (unsigned k = 0u; k < 8u; ++k) {
-          BR(R, r + 4u * k, p) = Gq[k];
-          BR(R, r + 4u * k, q) = Gp[k];
-          BR(W, r + 4u * k, p) = Vq[k];
-          BR(W, r + 4u * k, q) = Vp[k];
-        }
-      }
-      my_s += (unsigned)transf_s;
-      my_b += (unsigned)transf_b;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    // every lane of a slot holds the slot's counts: sum over the wave / 4
-    const unsigned swp_s = wave_sum_u(my_s) >> 2, swp_b = wave_sum_u(my_b) >> 2;
-    if (swp_s) {
-      blk_s += swp_s;
-      blk_b += swp_b;
-    }
-    else
-      break;
-  }
+  double cos_unused = 0.0;
+  wave_level0<SORT, false>(R, W, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, cos_unused);
   out_s = blk_s;
   out_b = blk_b;
   if (SORT || blk_s) {

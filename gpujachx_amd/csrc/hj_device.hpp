@@ -220,6 +220,163 @@ __device__ __forceinline__ void chol32_rows(double (&a)[32], unsigned x, double 
   }
 }
 
+// LDS of one wave's pair state: the 32x32 matrices R and W column-major with
+// leading dimension 33
+constexpr unsigned L0LD = 33u;
+#define L0A(A, r, c) (A)[(c) * L0LD + (r)]
+
+// the LDS accesses of a wave before this point complete before any after it
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum of the 32 rows of one pivot pair held by 4 lanes, lane r holding rows
+// r + 4k as v[k]: the xor butterfly 16,8,4,2,1 of the reference's 32-lane
+// reduction (levels 16, 8 and 4 inside the lane, 2 and 1 across lanes) --
+// the bits of sum8x4 on the same rows.
+__device__ __forceinline__ double sum4x8(const double *v)
+{
+  const double a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
+  double s = (a0 + a2) + (a1 + a3);
+  s = s + lane_xor2(s);
+  s = s + lane_xor1(s);
+  return s;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v += (unsigned)__shfl_xor((int)v, o, 64);
+  return v;
+}
+
+// Level-0 sweeps of one block pair by one wave (the inner Jacobi of
+// share/device_code_nosort.hpp:4-222 for SORT false,
+// share/device_code_cdsort_KISS.hpp:4-260 and share/device_code_cdsort.hpp:25-167
+// for SORT true).  R: the Cholesky factor, W: the identity on entry, both in
+// the L0A layout.  Slot y = l/4 takes pivot pair y of a step, lane r = l%4 of
+// the slot holds rows r + 4k of the two columns; the pivot pairs of a step
+// touch disjoint columns, so one wave-level LDS sync per step suffices.
+// Returns the rotation counts of all its sweeps (the reference's counting:
+// a sweep without a rotation ends the loop and adds nothing) in every lane,
+// and with COS the largest |A_pq| / sqrt(A_pp A_qq) over the pivots of the
+// first sweep.
+template <bool SORT, bool COS>
+__device__ __forceinline__ void wave_level0(double *R, double *W, const uint8_t *S0,
+                                            unsigned s0steps, unsigned nswp, int def, bool hyp,
+                                            unsigned npos, unsigned l, unsigned &out_s,
+                                            unsigned &out_b, double &out_cos)
+{
+  const unsigned y = l >> 2, r = l & 3u;
+  unsigned blk_s = 0u, blk_b = 0u;
+  double my_cos = 0.0;
+  for (unsigned swp = 0u; swp < nswp; ++swp) {
+    unsigned my_s = 0u, my_b = 0u;
+    for (unsigned step = 0u; step < s0steps; ++step) {
+      const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
+      double Gp[8], Gq[8];
+#pragma unroll
+      for (unsigned k = 0u; k < 8u; ++k) {
+        Gp[k] = L0A(R, r + 4u * k, p);
+        Gq[k] = L0A(R, r + 4u * k, q);
+      }
+      double Dp, Dq, Apq;
+      {
+        double t[8];
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k)
+          t[k] = __fma_rn(Gp[k], Gp[k], 0.0);
+        Dp = sum4x8(t);
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k)
+          t[k] = __fma_rn(Gq[k], Gq[k], 0.0);
+        Dq = sum4x8(t);
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k)
+          t[k] = __fma_rn(Gp[k], Gq[k], 0.0);
+        Apq = sum4x8(t);
+      }
+      const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
+      const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
+      if (COS && swp == 0u)
+        my_cos = fmax(my_cos, (Dpq_ > 0.0) ? (Apq_ / Dpq_) : 0.0);
+      int pdef = def;
+      if (hyp) {
+        if ((p < npos) && (q < npos)) pdef = +1;
+        else if ((p >= npos) && (q >= npos)) pdef = -1;
+        else pdef = 0;
+      }
+      int transf_b = 0;
+      // W's columns p, q (the same lane reads and writes the same entries
+      // within a step, so they may be read this late)
+      double Vp[8], Vq[8];
+#pragma unroll
+      for (unsigned k = 0u; k < 8u; ++k) {
+        Vp[k] = L0A(W, r + 4u * k, p);
+        Vq[k] = L0A(W, r + 4u * k, q);
+      }
+      if (transf_s) {
+        double c, t, t_;
+        bool trig;
+        if (!SORT) trig = !hyp || pdef;
+        else trig = (pdef != 0);
+        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
+        t_ = trig ? -t : t;
+        if (SORT && trig) {
+          Dp = __fma_rn(t, Apq, Dp);
+          Dq = __fma_rn(t_, Apq, Dq);
+        }
+        bool keep = true;
+        if (SORT)
+          keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
+        const unsigned PP = keep ? p : q, QQ = keep ? q : p;
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k) {
+          // big rotation: scale by c; otherwise (c == 1) the fma result as is
+          const double rp = __fma_rn(t_, Gq[k], Gp[k]), rq = __fma_rn(t, Gp[k], Gq[k]);
+          const double wp = __fma_rn(t_, Vq[k], Vp[k]), wq = __fma_rn(t, Vp[k], Vq[k]);
+          L0A(R, r + 4u * k, PP) = transf_b ? c * rp : rp;
+          L0A(R, r + 4u * k, QQ) = transf_b ? c * rq : rq;
+          L0A(W, r + 4u * k, PP) = transf_b ? c * wp : wp;
+          L0A(W, r + 4u * k, QQ) = transf_b ? c * wq : wq;
+        }
+      }
+      else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
+#pragma unroll
+        for (unsigned k = 0u; k < 8u; ++k) {
+          L0A(R, r + 4u * k, p) = Gq[k];
+          L0A(R, r + 4u * k, q) = Gp[k];
+          L0A(W, r + 4u * k, p) = Vq[k];
+          L0A(W, r + 4u * k, q) = Vp[k];
+        }
+      }
+      my_s += (unsigned)transf_s;
+      my_b += (unsigned)transf_b;
+      wave_lds_sync();
+    }
+    // every lane of a slot holds the slot's counts: sum over the wave / 4
+    const unsigned swp_s = wave_sum_u(my_s) >> 2, swp_b = wave_sum_u(my_b) >> 2;
+    if (swp_s) {
+      blk_s += swp_s;
+      blk_b += swp_b;
+    }
+    else
+      break;
+  }
+  out_s = blk_s;
+  out_b = blk_b;
+  if (COS) {
+#pragma unroll
+    for (int o = 32; o >= 4; o >>= 1)
+      my_cos = fmax(my_cos, __shfl_xor(my_cos, o, 64));
+  }
+  out_cos = my_cos;
+}
+
 }  // namespace hj
 
 #endif

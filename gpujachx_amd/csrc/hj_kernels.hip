@@ -323,37 +323,48 @@ __global__ __launch_bounds__(256, 2) void k_gram_split(const double *__restrict_
 #define RA(A, r, c) (A)[(c) * LDR + (r)]
 #define F32(A, r, c) (A)[(c) * 32u + (r)]
 
-template <bool SORT>
-__global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs,
-                                              const double *__restrict__ part, unsigned nchunk,
-                                              double *__restrict__ Wout, unsigned *__restrict__ flags,
-                                              RunCtl *__restrict__ ctl, VisitRec *__restrict__ rec,
-                                              const uint8_t *__restrict__ skip,
-                                              const uint8_t *__restrict__ strat0, unsigned s0steps,
-                                              unsigned nswp, int definite, unsigned nplus)
-{
-  __shared__ double R[32u * LDR];
-  __shared__ double W[32u * LDR];
-  __shared__ uint8_t S0[STRAT0_TAB_BYTES];
-  __shared__ unsigned cnt_s[16], cnt_b[16];
-  __shared__ float cmax[16];
-  __shared__ float bnorm[2];
-  __shared__ int chol_ok;
-  __shared__ double cholcol[32];
+// LDS of one pair visit (128 threads)
+struct PairLds {
+  double R[32u * LDR];
+  double W[32u * LDR];
+  double cholcol[32];
+  unsigned cnt_s[16], cnt_b[16];
+  float cmax[16];
+  float bnorm[2];
+  int chol_ok;
+  int go;     // the visit still runs inner sweeps
+  int ready;  // dataflow sweep: the visit's inputs are complete
+};
 
-  const unsigned tid = threadIdx.x;
-  const unsigned pi = blockIdx.x;
-  const uint8_t plan = skip ? skip[pi] : PLAN_FP64;
-  if (plan == PLAN_SKIP) {
+// One pair visit by the 128 threads tid = 0..127 of a workgroup that may
+// run two visits side by side (256 threads, one PairLds each): both halves
+// execute the same barriers, a half without work (no pair, a skipped visit,
+// a failed split Cholesky, converged inner sweeps) passes them idle.
+// go_other: the other half's go flag, or null when the workgroup holds one
+// visit.  plan: the visit's plan (PLAN_FP64 without a plan).
+template <bool SORT>
+__device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned pi, PairLds &L,
+                                           const int *go_other, const uint16_t *__restrict__ pairs,
+                                           const double *__restrict__ part, unsigned nchunk,
+                                           double *__restrict__ Wout, unsigned *__restrict__ flags,
+                                           RunCtl *__restrict__ ctl, VisitRec *__restrict__ rec,
+                                           uint8_t plan, const uint8_t *S0, unsigned s0steps,
+                                           unsigned nswp, int definite, unsigned nplus)
+{
+  double *const R = L.R;
+  double *const W = L.W;
+  if (present && plan == PLAN_SKIP && tid == 0u) {
     // skipped visit: no update, the record keeps the last executed visit
-    if (tid == 0u) {
-      flags[pi] = 0u;
-      rec[pi].skips += 1u;
-      atomicAdd(&ctl->skipped, 1u);
-    }
-    return;
+    flags[pi] = 0u;
+    rec[pi].skips += 1u;
+    atomicAdd(&ctl->skipped, 1u);
   }
-  const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
+  bool live = present && plan != PLAN_SKIP;
+  unsigned iBlk = 0u, jBlk = 0u;
+  if (live) {
+    iBlk = pairs[2u * pi];
+    jBlk = pairs[2u * pi + 1u];
+  }
 
   // local J structure, share/device_code_nosort_accumV.hpp:53-76
   bool hyp = false;
@@ -369,58 +380,55 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
     hyp = (def == 0);
   }
 
-  // the level-0 table (STRAT0_TAB_BYTES, 16-byte aligned) in one pass of
-  // 16-byte loads
-  if (tid < STRAT0_TAB_BYTES / 16u)
-    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(strat0)[tid];
-
-  // sum of the chunk partials in chunk order: three entries per thread at a
-  // time, all their chunk loads in flight together
-  const double *const pp = part + (size_t)pi * nchunk * 768u;
+  if (live) {
+    // sum of the chunk partials in chunk order: three entries per thread at
+    // a time, all their chunk loads in flight together
+    const double *const pp = part + (size_t)pi * nchunk * 768u;
 #pragma unroll 1
-  for (unsigned e0 = tid; e0 < 768u; e0 += 384u) {
-    double v[3][MAX_GRAM_CHUNKS];
+    for (unsigned e0 = tid; e0 < 768u; e0 += 384u) {
+      double v[3][MAX_GRAM_CHUNKS];
 #pragma unroll
-    for (unsigned q = 0u; q < 3u; ++q)
+      for (unsigned q = 0u; q < 3u; ++q)
 #pragma unroll
-      for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
-        if (c < nchunk) v[q][c] = pp[(size_t)c * 768u + e0 + 128u * q];
+        for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
+          if (c < nchunk) v[q][c] = pp[(size_t)c * 768u + e0 + 128u * q];
 #pragma unroll
-    for (unsigned q = 0u; q < 3u; ++q) {
-    const unsigned e = e0 + 128u * q;
-    const unsigned tile = e >> 8;
-    double s = v[q][0];
+      for (unsigned q = 0u; q < 3u; ++q) {
+        const unsigned e = e0 + 128u * q;
+        const unsigned tile = e >> 8;
+        double s = v[q][0];
 #pragma unroll
-    for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
-      if (c < nchunk) s = s + v[q][c];
-    const unsigned l = (e >> 2) & 63u, r = e & 3u;
-    const unsigned i = 4u * r + (l >> 4), j = l & 15u;
-    const unsigned row = (tile == 0u) ? i : (16u + i);
-    const unsigned col = (tile == 2u) ? (16u + j) : j;
-    RA(R, row, col) = s;
+        for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
+          if (c < nchunk) s = s + v[q][c];
+        const unsigned l = (e >> 2) & 63u, r = e & 3u;
+        const unsigned i = 4u * r + (l >> 4), j = l & 15u;
+        const unsigned row = (tile == 0u) ? i : (16u + i);
+        const unsigned col = (tile == 2u) ? (16u + j) : j;
+        RA(R, row, col) = s;
+      }
     }
+    for (unsigned e = tid; e < 1024u; e += 128u)
+      RA(W, e & 31u, e >> 5) = ((e & 31u) == (e >> 5)) ? 1.0 : 0.0;
   }
-  for (unsigned e = tid; e < 1024u; e += 128u)
-    RA(W, e & 31u, e >> 5) = ((e & 31u) == (e >> 5)) ? 1.0 : 0.0;
   __syncthreads();
-  if (tid < 2u) {
+  if (live && tid < 2u) {
     // squared Frobenius norms of the two blocks, for the visit record
     double sn = 0.0;
     for (unsigned k = 0u; k < 16u; ++k)
       sn = sn + RA(R, 16u * tid + k, 16u * tid + k);
-    bnorm[tid] = (float)log2(sn);
+    L.bnorm[tid] = (float)log2(sn);
   }
 
   // Cholesky, share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT):
   // column k scaled by rsqrt(A(k,k)), diagonal sqrt(A(k,k)), then
   // A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k; R = L^T.
-  if (tid < 64u) {
+  if (live && tid < 64u) {
     const unsigned x = tid & 31u;
     double a[32];
 #pragma unroll
     for (unsigned j = 0u; j < 32u; ++j)
       a[j] = (j <= x) ? RA(R, x, j) : 0.0;
-    chol32_rows(a, x, cholcol);
+    chol32_rows(a, x, L.cholcol);
     if (tid < 32u) {
 #pragma unroll
       for (unsigned y = 0u; y < 32u; ++y)
@@ -436,26 +444,26 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
         if (j == x) ok = ok && (a[j] > 0.0);
       }
       const unsigned long long good = __ballot(ok);
-      if (tid == 0u) chol_ok = (good == ~0ull) ? 1 : 0;
+      if (tid == 0u) L.chol_ok = (good == ~0ull) ? 1 : 0;
     }
   }
   __syncthreads();
-  if (plan == PLAN_SPLIT && !chol_ok) {
+  if (live && plan == PLAN_SPLIT && !L.chol_ok) {
     if (tid == 0u) {
       flags[pi] = 0u;
       VisitRec v;
       v.rot_s = 0u;
       v.rot_b = 0u;
       v.maxcos = 1.0f;
-      v.l2ni = bnorm[0];
-      v.l2nj = bnorm[1];
+      v.l2ni = L.bnorm[0];
+      v.l2nj = L.bnorm[1];
       v.sweep = ctl->sweep;
       v.skips = 0u;
       v.flags = REC_SPLIT | REC_SPLIT_FAIL;
       rec[pi] = v;
       atomicAdd(&ctl->split, 1u);
     }
-    return;
+    live = false;
   }
 
   // Inner Jacobi: share/device_code_nosort.hpp:4-222 (SORT false),
@@ -465,9 +473,16 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
   const unsigned y = tid >> 3, r = tid & 7u;
   unsigned blk_s = 0u, blk_b = 0u;
   double my_cos = 0.0;
+  bool sweeping = live;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
+    // the halves go on while either still sweeps
+    if (tid == 0u) L.go = sweeping ? 1 : 0;
+    __syncthreads();
+    if (!L.go && !(go_other && *go_other))
+      break;
     unsigned my_s = 0u, my_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
+      if (sweeping) {
       const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
       double Gp[4], Gq[4], Vp[4], Vq[4];
 #pragma unroll
@@ -531,30 +546,35 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
       }
       my_s += (unsigned)transf_s;
       my_b += (unsigned)transf_b;
+      }
       __syncthreads();
     }
-    if (r == 0u) {
-      cnt_s[y] = my_s;
-      cnt_b[y] = my_b;
+    if (sweeping && r == 0u) {
+      L.cnt_s[y] = my_s;
+      L.cnt_b[y] = my_b;
     }
     __syncthreads();
-    unsigned swp_s = 0u, swp_b = 0u;
-    for (unsigned i = 0u; i < 16u; ++i) {
-      swp_s += cnt_s[i];
-      swp_b += cnt_b[i];
+    if (sweeping) {
+      unsigned swp_s = 0u, swp_b = 0u;
+      for (unsigned i = 0u; i < 16u; ++i) {
+        swp_s += L.cnt_s[i];
+        swp_b += L.cnt_b[i];
+      }
+      if (swp_s) {
+        blk_s += swp_s;
+        blk_b += swp_b;
+      }
+      else
+        sweeping = false;
     }
     __syncthreads();
-    if (swp_s) {
-      blk_s += swp_s;
-      blk_b += swp_b;
-    }
-    else
-      break;
   }
 
-  if (r == 0u)
-    cmax[y] = (float)my_cos;
+  if (live && r == 0u)
+    L.cmax[y] = (float)my_cos;
   __syncthreads();
+  if (!live)
+    return;
   // convergence counter, share/device_code_nosort.hpp:94-103
   if (tid == 0u) {
     unsigned long long *const cvg = &ctl->cvg;
@@ -567,13 +587,13 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
     flags[pi] = SORT ? 1u : (blk_s ? 1u : 0u);
     float cm = 0.0f;
     for (unsigned i = 0u; i < 16u; ++i)
-      cm = fmaxf(cm, cmax[i]);
+      cm = fmaxf(cm, L.cmax[i]);
     VisitRec v;
     v.rot_s = blk_s;
     v.rot_b = blk_b;
     v.maxcos = cm;
-    v.l2ni = bnorm[0];
-    v.l2nj = bnorm[1];
+    v.l2ni = L.bnorm[0];
+    v.l2nj = L.bnorm[1];
     v.sweep = ctl->sweep;
     v.skips = 0u;
     v.flags = (plan == PLAN_SPLIT) ? REC_SPLIT : 0u;
@@ -584,6 +604,36 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
   double *const wo = Wout + (size_t)pi * 1024u;
   for (unsigned e = tid; e < 1024u; e += 128u)
     wo[e] = RA(W, e & 31u, e >> 5);
+}
+
+template <bool SORT>
+__global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs,
+                                              const double *__restrict__ part, unsigned nchunk,
+                                              double *__restrict__ Wout, unsigned *__restrict__ flags,
+                                              RunCtl *__restrict__ ctl, VisitRec *__restrict__ rec,
+                                              const uint8_t *__restrict__ skip,
+                                              const uint8_t *__restrict__ strat0, unsigned s0steps,
+                                              unsigned nswp, int definite, unsigned nplus)
+{
+  __shared__ PairLds L;
+  __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
+  const unsigned tid = threadIdx.x, pi = blockIdx.x;
+  const uint8_t plan = skip ? skip[pi] : PLAN_FP64;
+  if (plan == PLAN_SKIP) {
+    // skipped visit: no update, the record keeps the last executed visit
+    if (tid == 0u) {
+      flags[pi] = 0u;
+      rec[pi].skips += 1u;
+      atomicAdd(&ctl->skipped, 1u);
+    }
+    return;
+  }
+  // the level-0 table (STRAT0_TAB_BYTES, 16-byte aligned) in one pass of
+  // 16-byte loads
+  if (tid < STRAT0_TAB_BYTES / 16u)
+    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(strat0)[tid];
+  pair_visit<SORT>(tid, true, pi, L, nullptr, pairs, part, nchunk, Wout, flags, ctl, rec, plan, S0,
+                   s0steps, nswp, definite, nplus);
 }
 
 // ---------------------------------------------------------------------------
@@ -835,27 +885,39 @@ __device__ __forceinline__ void store_partial(double *o, unsigned l, bool split,
   }
 }
 
-// Every non-skipped next pair gets the FP64 Gram, whatever its plan: riding
-// on the update's HBM stream the Gram adds no memory traffic, and this
-// kernel is bound by HBM, not by the FP64 MFMA units.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
-void k_upgram(double *__restrict__ G, unsigned ldG,
-                                                const uint16_t *__restrict__ pairs,
-                                                const double *__restrict__ Wbuf,
-                                                const unsigned *__restrict__ flags,
-                                                const Quad *__restrict__ quads, unsigned nchunk,
-                                                unsigned CH, double *__restrict__ part,
-                                                const uint8_t *__restrict__ plan_next)
+// relaxed device-scope load (a vector load, coherent across the XCDs)
+__device__ __forceinline__ unsigned ld_relaxed(const unsigned *p)
 {
-  __shared__ d4 red[2][3][GRAM_WAVES - 1][64];
-  __shared__ double Wl[2][32u * LDW];
-  __shared__ double Tb[GRAM_WAVES][16u * LDT];
-  const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const unsigned qi = blockIdx.x / nchunk, c = blockIdx.x - qi * nchunk;
-  const Quad Q = quads[qi];
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS of one fused workgroup: wave partials, W of both pairs, the waves'
+// transpose buffers
+struct UpgLds {
+  d4 red[2][3][GRAM_WAVES - 1][64];
+  double Wl[2][32u * LDW];
+  double Tb[GRAM_WAVES][16u * LDT];
+};
+
+// One (quad, row chunk) of the fused transition by the workgroup's first nw
+// waves (the others only pass the barriers).  Every non-skipped next pair
+// gets the FP64 Gram, whatever its plan: riding on the update's HBM stream
+// the Gram adds no memory traffic, and the kernel is bound by HBM, not by
+// the FP64 MFMA units.
+__device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, unsigned ldG,
+                                            const uint16_t *__restrict__ pairs,
+                                            const double *__restrict__ Wbuf,
+                                            const unsigned *__restrict__ flags, const Quad &Q,
+                                            unsigned c, unsigned nchunk, unsigned CH,
+                                            double *__restrict__ part,
+                                            const uint8_t *__restrict__ plan_next, unsigned nw)
+{
+  const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const unsigned A0 = pairs[2u * Q.p0], B0 = pairs[2u * Q.p0 + 1u];
   const unsigned A1 = pairs[2u * Q.p1], B1 = pairs[2u * Q.p1 + 1u];
-  const bool u0 = flags[Q.p0] != 0u, u1 = flags[Q.p1] != 0u;
+  // the flags come from pair visits of the same launch in the dataflow
+  // sweep: atomic loads, never through the scalar cache
+  const bool u0 = ld_relaxed(flags + Q.p0) != 0u, u1 = ld_relaxed(flags + Q.p1) != 0u;
   uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
   uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
   if (!u0 && !u1 && pn0 == PLAN_SKIP && pn1 == PLAN_SKIP)
@@ -863,72 +925,257 @@ void k_upgram(double *__restrict__ G, unsigned ldG,
   // W(k, j) of both pairs into LDS (row-major, padded)
   for (unsigned e = threadIdx.x; e < 2048u; e += blockDim.x) {
     const unsigned pp = e >> 10, k = (e >> 5) & 31u, j = e & 31u;
-    Wl[pp][k * LDW + j] = F32(Wbuf + (size_t)(pp ? Q.p1 : Q.p0) * 1024u, k, j);
+    S.Wl[pp][k * LDW + j] = F32(Wbuf + (size_t)(pp ? Q.p1 : Q.p0) * 1024u, k, j);
   }
   __syncthreads();
   const unsigned SUB = CH / nw;
   const size_t rbeg = (size_t)c * CH + (size_t)w * SUB;
-  double *const Tw = Tb[w];
+  double *const Tw = S.Tb[w];
   d4 z = {0.0, 0.0, 0.0, 0.0};
   d4 g00 = z, g10 = z, g11 = z, h00 = z, h10 = z, h11 = z;
-  double xa[8], xb[8];
+  if (w < nw) {
+    double xa[8], xb[8];
 #if UPGRAM_PREFETCH
-  slab_load(G, ldG, rbeg, A0, B0, u0, l, xa);
-  slab_load(G, ldG, rbeg, A1, B1, u1, l, xb);
+    slab_load(G, ldG, rbeg, A0, B0, u0, l, xa);
+    slab_load(G, ldG, rbeg, A1, B1, u1, l, xb);
 #endif
-  for (unsigned g = 0u; g < SUB; g += 16u) {
-    d4 t0, t1, t2, t3;
+    for (unsigned g = 0u; g < SUB; g += 16u) {
+      d4 t0, t1, t2, t3;
 #if UPGRAM_PREFETCH
-    double ca[8], cb[8];
+      double ca[8], cb[8];
 #pragma unroll
-    for (unsigned u = 0u; u < 8u; ++u) {
-      ca[u] = xa[u];
-      cb[u] = xb[u];
-    }
-    if (g + 16u < SUB) {
-      slab_load(G, ldG, rbeg + g + 16u, A0, B0, u0, l, xa);
-      slab_load(G, ldG, rbeg + g + 16u, A1, B1, u1, l, xb);
-    }
-    slab_compute(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, ca, t0, t1, Tw);
-    slab_compute(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, cb, t2, t3, Tw);
+      for (unsigned u = 0u; u < 8u; ++u) {
+        ca[u] = xa[u];
+        cb[u] = xb[u];
+      }
+      if (g + 16u < SUB) {
+        slab_load(G, ldG, rbeg + g + 16u, A0, B0, u0, l, xa);
+        slab_load(G, ldG, rbeg + g + 16u, A1, B1, u1, l, xb);
+      }
+      slab_compute(G, ldG, rbeg + g, A0, B0, u0, S.Wl[0], l, ca, t0, t1, Tw);
+      slab_compute(G, ldG, rbeg + g, A1, B1, u1, S.Wl[1], l, cb, t2, t3, Tw);
 #else
-    slab_load(G, ldG, rbeg + g, A0, B0, u0, l, xa);
-    slab_load(G, ldG, rbeg + g, A1, B1, u1, l, xb);
-    slab_compute(G, ldG, rbeg + g, A0, B0, u0, Wl[0], l, xa, t0, t1, Tw);
-    slab_compute(G, ldG, rbeg + g, A1, B1, u1, Wl[1], l, xb, t2, t3, Tw);
+      slab_load(G, ldG, rbeg + g, A0, B0, u0, l, xa);
+      slab_load(G, ldG, rbeg + g, A1, B1, u1, l, xb);
+      slab_compute(G, ldG, rbeg + g, A0, B0, u0, S.Wl[0], l, xa, t0, t1, Tw);
+      slab_compute(G, ldG, rbeg + g, A1, B1, u1, S.Wl[1], l, xb, t2, t3, Tw);
 #endif
-    if (pn0 != PLAN_SKIP) {
-      const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
-      gram_fp64_slab(X, Y, g00, g10, g11);
+      if (pn0 != PLAN_SKIP) {
+        const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
+        gram_fp64_slab(X, Y, g00, g10, g11);
+      }
+      if (pn1 != PLAN_SKIP) {
+        const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
+        gram_fp64_slab(X, Y, h00, h10, h11);
+      }
     }
-    if (pn1 != PLAN_SKIP) {
-      const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
-      gram_fp64_slab(X, Y, h00, h10, h11);
+    if (w && w < nw) {
+      S.red[0][0][w - 1u][l] = g00;
+      S.red[0][1][w - 1u][l] = g10;
+      S.red[0][2][w - 1u][l] = g11;
+      S.red[1][0][w - 1u][l] = h00;
+      S.red[1][1][w - 1u][l] = h10;
+      S.red[1][2][w - 1u][l] = h11;
     }
-  }
-  if (w) {
-    red[0][0][w - 1u][l] = g00;
-    red[0][1][w - 1u][l] = g10;
-    red[0][2][w - 1u][l] = g11;
-    red[1][0][w - 1u][l] = h00;
-    red[1][1][w - 1u][l] = h10;
-    red[1][2][w - 1u][l] = h11;
   }
   __syncthreads();
   if (w == 0u) {
     for (unsigned v = 1u; v < nw; ++v) {
-      g00 = g00 + red[0][0][v - 1u][l];
-      g10 = g10 + red[0][1][v - 1u][l];
-      g11 = g11 + red[0][2][v - 1u][l];
-      h00 = h00 + red[1][0][v - 1u][l];
-      h10 = h10 + red[1][1][v - 1u][l];
-      h11 = h11 + red[1][2][v - 1u][l];
+      g00 = g00 + S.red[0][0][v - 1u][l];
+      g10 = g10 + S.red[0][1][v - 1u][l];
+      g11 = g11 + S.red[0][2][v - 1u][l];
+      h00 = h00 + S.red[1][0][v - 1u][l];
+      h10 = h10 + S.red[1][1][v - 1u][l];
+      h11 = h11 + S.red[1][2][v - 1u][l];
     }
     if (pn0 != PLAN_SKIP)
       store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, false, g00, g10, g11);
     if (pn1 != PLAN_SKIP)
       store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, false, h00, h10, h11);
   }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
+void k_upgram(double *__restrict__ G, unsigned ldG, const uint16_t *__restrict__ pairs,
+              const double *__restrict__ Wbuf, const unsigned *__restrict__ flags,
+              const Quad *__restrict__ quads, unsigned nchunk, unsigned CH,
+              double *__restrict__ part, const uint8_t *__restrict__ plan_next)
+{
+  __shared__ UpgLds S;
+  const unsigned qi = blockIdx.x / nchunk, c = blockIdx.x - qi * nchunk;
+  const Quad Q = quads[qi];
+  upgram_item(S, G, ldG, pairs, Wbuf, flags, Q, c, nchunk, CH, part, plan_next, blockDim.x >> 6);
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow sweep: all steps of a sweep in one persistent launch.  The
+// launch's work is a list of items in an order where every item's inputs
+// come from earlier items:
+//   UPGRAM(s, quad, chunk): the fused transition s -> s+1 of one closed quad
+//     over one row chunk (upgram_item), after the pair visits of step s of
+//     the quad's two pairs;
+//   PAIRS(s, up to 2 pairs): 128 threads per pair, the visit of k_pair
+//     (pair_visit: sum of the Gram partials, Cholesky, level-0 sweeps, W),
+//     after every chunk of the quad of step s-1 whose next pairs include it.
+// A workgroup takes the next item by an atomic ticket, waits (one lane,
+// bounded in time) until the counters of the item's inputs are complete,
+// runs it, and counts it done.  An item waits only on items of smaller
+// tickets, each already held by a running workgroup, so the waits end.
+// W, the update flags and the Gram partials live in rings of R steps; an
+// item that overwrites a ring slot first waits until the items of the step
+// that last used the slot are done.  The results are bit for bit those of
+// the step-by-step launches: every item runs the arithmetic of k_upgram /
+// k_pair on the same data.
+
+// the waits give up after this many 100 MHz clock ticks (2 s) and raise the
+// run's error word; every later wait then fails at once
+constexpr unsigned long long FLOW_TIMEOUT = 200000000ull;
+
+// the wave's later loads see what the producers released before the counts
+// this wave (or its workgroup, through a barrier) has observed
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// one lane: wait until *p >= v (relaxed polling, one acquire fence at the
+// end); false when the run has failed
+__device__ bool flow_wait(const unsigned *p, unsigned v, unsigned *err)
+{
+  if (ld_relaxed(p) < v) {
+    const unsigned long long t0 = wall_clock64();
+    for (;;) {
+      __builtin_amdgcn_s_sleep(2);
+      if (ld_relaxed(p) >= v)
+        break;
+      if (ld_relaxed(err))
+        return false;
+      if (wall_clock64() - t0 > FLOW_TIMEOUT) {
+        atomicOr(err, 1u);
+        return false;
+      }
+    }
+  }
+  acquire_agent();
+  return true;
+}
+
+// pair_visit as a call of its own in the dataflow kernel: its register
+// allocation stays apart from the fused items'
+template <bool SORT>
+__device__ __noinline__ void flow_pair_visit(unsigned tid, bool present, unsigned pi, PairLds &L,
+                                             const int *go_other, const uint16_t *pairs,
+                                             const double *part, unsigned nchunk, double *Wout,
+                                             unsigned *flags, RunCtl *ctl, VisitRec *rec,
+                                             uint8_t plan, const uint8_t *S0, unsigned s0steps,
+                                             unsigned nswp, int definite, unsigned nplus)
+{
+  pair_visit<SORT>(tid, present, pi, L, go_other, pairs, part, nchunk, Wout, flags, ctl, rec, plan,
+                   S0, s0steps, nswp, definite, nplus);
+}
+
+// LDS of a flow workgroup: the fused item's, or two pair visits'
+union FlowLds {
+  UpgLds upg;
+  PairLds pair[2];
+};
+
+template <bool SORT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
+void k_flow(FlowArgs A)
+{
+  __shared__ FlowLds S;
+  __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
+  __shared__ unsigned s_item, s_ok;
+  const unsigned tid = threadIdx.x;
+  unsigned *const err = &A.ctl->flow_err;
+  if (tid < STRAT0_TAB_BYTES / 16u)
+    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(A.strat0)[tid];
+  for (;;) {
+    if (tid == 0u)
+      s_item = atomicAdd(A.cnt, 1u);
+    __syncthreads();
+    const unsigned t = s_item;
+    if (t >= A.nitems)
+      break;
+    const FlowItem it = A.items[t];
+    const unsigned st = it.step;
+    const unsigned ring = st % A.R;
+    if (A.trace && tid == 0u)
+      A.trace[3u * (size_t)t] = wall_clock64();
+    if (it.kind == FLOW_UPGRAM) {
+      const Quad Q = A.quads[(size_t)st * A.nq + it.a[0]];
+      const unsigned *const pd = A.pdone + (size_t)st * A.np;
+      if (tid == 0u) {
+        // W and flags of both pairs; the partial slots of step st+1 free
+        bool ok = flow_wait(pd + Q.p0, 1u, err) && flow_wait(pd + Q.p1, 1u, err);
+        if (ok && st + 1u >= A.R)
+          ok = flow_wait(A.spg + (st + 1u - A.R), A.np, err);
+        s_ok = ok ? 1u : 0u;
+        if (A.trace)
+          A.trace[3u * (size_t)t + 1u] = wall_clock64();
+      }
+      __syncthreads();
+      if (s_ok) {
+        acquire_agent();
+        const unsigned nr = (st + 1u) % A.R;
+        upgram_item(S.upg, A.G, A.ldG, A.s1 + (size_t)st * A.np * 2u,
+                    A.W + (size_t)ring * A.np * 1024u, A.flags + (size_t)ring * A.np, Q, it.a[1],
+                    A.nchunk, A.CH, A.part + (size_t)nr * A.np * A.nchunk * 768u,
+                    A.plan ? A.plan + (size_t)(st + 1u) * A.np : nullptr, A.nw);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0u) {
+          atomicAdd(A.qdone + (size_t)st * A.nq + it.a[0], 1u);
+          atomicAdd(A.supg + st, 1u);
+        }
+      }
+    }
+    else {
+      // two pair visits, one per half of the workgroup
+      const unsigned half = tid >> 7, ht = tid & 127u;
+      const unsigned pi = it.a[half];
+      if (ht == 0u) {
+        bool ok = pi != FLOW_NONE;
+        // the partials of the pair; the W / flag slots of step st free
+        if (ok && st)
+          ok = flow_wait(A.qdone + (size_t)(st - 1u) * A.nq + A.srcq[(size_t)st * A.np + pi],
+                         A.nchunk, err);
+        if (ok && st >= A.R)
+          ok = flow_wait(A.supg + (st - A.R), A.nq * A.nchunk, err);
+        S.pair[half].ready = ok ? 1 : 0;
+        if (A.trace && tid == 0u)
+          A.trace[3u * (size_t)t + 1u] = wall_clock64();
+      }
+      __syncthreads();
+      const bool present = S.pair[half].ready != 0;
+      __syncthreads();
+      acquire_agent();
+      flow_pair_visit<SORT>(ht, present, pi, S.pair[half], &S.pair[half ^ 1u].go,
+                       A.s1 + (size_t)st * A.np * 2u,
+                       A.part + (size_t)ring * A.np * A.nchunk * 768u, A.nchunk,
+                       A.W + (size_t)ring * A.np * 1024u, A.flags + (size_t)ring * A.np, A.ctl,
+                       A.rec + (size_t)st * A.np,
+                       (present && A.plan) ? A.plan[(size_t)st * A.np + pi] : PLAN_FP64, S0,
+                       A.s0steps, A.nswp, A.definite, A.nplus);
+      __threadfence();
+      __syncthreads();
+      if (present && ht == 0u) {
+        __hip_atomic_store(A.pdone + (size_t)st * A.np + pi, 1u, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(A.spg + st, 1u);
+      }
+    }
+    __syncthreads();
+    if (A.trace && tid == 0u)
+      A.trace[3u * (size_t)t + 2u] = wall_clock64();
+  }
+}
+
+// counters of a flow launch to zero (a kernel, so the sweep graph holds no
+// memset node)
+__global__ void k_flow_reset(unsigned *cnt, size_t n)
+{
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    cnt[e] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_initD(double *__restrict__ G, double *__restrict__ D,
@@ -1032,6 +1279,34 @@ void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs
   const unsigned nw = gram_waves(CH);
   hipLaunchKernelGGL(k_upgram, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W, flags,
                      quads, nchunk, CH, part, plan_next);
+}
+
+void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
+{
+  if (sort)
+    hipLaunchKernelGGL(k_flow<true>, dim3(nblocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_flow<false>, dim3(nblocks), dim3(256), 0, s, a);
+}
+
+unsigned flow_blocks_per_cu()
+{
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<false>, 256, 0) != hipSuccess || n < 1)
+    n = 1;
+  int m = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_flow<true>, 256, 0) == hipSuccess && m >= 1 &&
+      m < n)
+    n = m;
+  return (unsigned)n;
+}
+
+void launch_flow_reset(hipStream_t s, unsigned *cnt, size_t n)
+{
+  size_t blocks = (n + 255u) / 256u;
+  if (blocks > 1024u) blocks = 1024u;
+  if (!blocks) blocks = 1u;
+  hipLaunchKernelGGL(k_flow_reset, dim3((unsigned)blocks), dim3(256), 0, s, cnt, n);
 }
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,

@@ -237,7 +237,8 @@ void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
 
 /* Kernel timing of the last call (HIP events on the work stream, enabled
  * by hj_set_profile(1); graphs are not used while profiling):
- * kind 0 = Gram, 1 = pair, 2 = update; returns total ms and launch count. */
+ * kind 0 = Gram, 1 = pair, 2 = update, 3 = the dataflow sweep kernel;
+ * returns total ms and launch count. */
 void hj_set_profile(int on);
 double hj_kernel_ms(int kind, unsigned long long *launches);
 
@@ -252,6 +253,19 @@ unsigned hj_set_max_sweeps(unsigned n);
 
 /* Test limit: stop after `steps` level-1 steps in total (0 = no limit). */
 void hj_set_step_limit(long steps);
+
+/* Dataflow sweeps (no counterpart in the reference, whose sweep is one
+ * kernel launch per step, 1/HypJacL2.cu:131-148): every step of a sweep in
+ * one persistent launch whose pair visits run while other pairs' block
+ * updates stream through HBM, the steps ordered by device-side counters
+ * instead of launch boundaries.  The results are bit for bit those of the
+ * step-by-step launches.  Used when on, the strategy pairs every step
+ * transition into closed quads (cycwor, rowcyc, colcyc, cycloc), V is not
+ * accumulated and no step limit is set; otherwise the step-by-step path
+ * runs.  Returns the previous setting.  hj_last_dataflow() tells whether
+ * the last hypJacL2 / hypJacL2_dev call used them. */
+int hj_set_dataflow(int on);
+int hj_last_dataflow(void);
 
 /* Device self-test of the scalar kernels: for n inputs, kind 0 = rsqrt(a),
  * 1 = trigonometric rotation (a=Apq,b=Dp,c=Dq), 2 = hyperbolic rotation.
