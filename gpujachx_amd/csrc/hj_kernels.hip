@@ -771,7 +771,8 @@ __device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
 // unchanged slab in the Gram layout (x[r], x[4 + r]: row 4r + l/16 of
 // column l%16 of blocks a, b).
 __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned ldG, size_t row,
-                                          unsigned a, unsigned b, bool upd, unsigned l, double *x)
+                                          unsigned a, unsigned b, bool upd, unsigned l, double *x,
+                                          bool la = true, bool lb = true)
 {
   if (upd) {
 #pragma unroll
@@ -782,12 +783,13 @@ __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned
     }
   }
   else {
+    // la / lb false: no next pair's Gram wants block a / b, nothing to read
     const double *const ia = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
     const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
 #pragma unroll
     for (unsigned r = 0u; r < 4u; ++r) {
-      x[r] = UG_LOAD(ia + 4u * r);
-      x[4u + r] = UG_LOAD(ib + 4u * r);
+      x[r] = la ? UG_LOAD(ia + 4u * r) : 0.0;
+      x[4u + r] = lb ? UG_LOAD(ib + 4u * r) : 0.0;
     }
   }
 }
@@ -922,6 +924,12 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
   uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
   if (!u0 && !u1 && pn0 == PLAN_SKIP && pn1 == PLAN_SKIP)
     return;  // nothing to update, no Gram wanted
+  // blocks of a pair without an update are read only for a next pair's Gram
+  const bool g0 = pn0 != PLAN_SKIP, g1 = pn1 != PLAN_SKIP;
+  const bool nt0 = (g0 && (Q.x0 == 0u || Q.y0 == 0u)) || (g1 && (Q.x1 == 0u || Q.y1 == 0u));
+  const bool nt1 = (g0 && (Q.x0 == 1u || Q.y0 == 1u)) || (g1 && (Q.x1 == 1u || Q.y1 == 1u));
+  const bool nt2 = (g0 && (Q.x0 == 2u || Q.y0 == 2u)) || (g1 && (Q.x1 == 2u || Q.y1 == 2u));
+  const bool nt3 = (g0 && (Q.x0 == 3u || Q.y0 == 3u)) || (g1 && (Q.x1 == 3u || Q.y1 == 3u));
   // W(k, j) of both pairs into LDS (row-major, padded)
   for (unsigned e = threadIdx.x; e < 2048u; e += blockDim.x) {
     const unsigned pp = e >> 10, k = (e >> 5) & 31u, j = e & 31u;
@@ -936,8 +944,8 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
   if (w < nw) {
     double xa[8], xb[8];
 #if UPGRAM_PREFETCH
-    slab_load(G, ldG, rbeg, A0, B0, u0, l, xa);
-    slab_load(G, ldG, rbeg, A1, B1, u1, l, xb);
+    slab_load(G, ldG, rbeg, A0, B0, u0, l, xa, nt0, nt1);
+    slab_load(G, ldG, rbeg, A1, B1, u1, l, xb, nt2, nt3);
 #endif
     for (unsigned g = 0u; g < SUB; g += 16u) {
       d4 t0, t1, t2, t3;
@@ -949,14 +957,14 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
         cb[u] = xb[u];
       }
       if (g + 16u < SUB) {
-        slab_load(G, ldG, rbeg + g + 16u, A0, B0, u0, l, xa);
-        slab_load(G, ldG, rbeg + g + 16u, A1, B1, u1, l, xb);
+        slab_load(G, ldG, rbeg + g + 16u, A0, B0, u0, l, xa, nt0, nt1);
+        slab_load(G, ldG, rbeg + g + 16u, A1, B1, u1, l, xb, nt2, nt3);
       }
       slab_compute(G, ldG, rbeg + g, A0, B0, u0, S.Wl[0], l, ca, t0, t1, Tw);
       slab_compute(G, ldG, rbeg + g, A1, B1, u1, S.Wl[1], l, cb, t2, t3, Tw);
 #else
-      slab_load(G, ldG, rbeg + g, A0, B0, u0, l, xa);
-      slab_load(G, ldG, rbeg + g, A1, B1, u1, l, xb);
+      slab_load(G, ldG, rbeg + g, A0, B0, u0, l, xa, nt0, nt1);
+      slab_load(G, ldG, rbeg + g, A1, B1, u1, l, xb, nt2, nt3);
       slab_compute(G, ldG, rbeg + g, A0, B0, u0, S.Wl[0], l, xa, t0, t1, Tw);
       slab_compute(G, ldG, rbeg + g, A1, B1, u1, S.Wl[1], l, xb, t2, t3, Tw);
 #endif
