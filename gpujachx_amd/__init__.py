@@ -92,6 +92,8 @@ def lib():
     L.hj_set_step_limit.restype = None
     L.hj_set_dataflow.argtypes = [i]
     L.hj_set_dataflow.restype = i
+    L.hj_set_factorization.argtypes = [i]
+    L.hj_set_factorization.restype = i
     L.hj_last_dataflow.argtypes = []
     L.hj_last_dataflow.restype = i
     L.hj_dbg_scalar.argtypes = [i, i, pd, pd, pd, pd, pd, ctypes.POINTER(ctypes.c_int)]
@@ -267,6 +269,16 @@ def set_dataflow(on):
     """Dataflow sweeps on/off (one persistent launch per sweep where the
     strategy and routine allow it); returns the previous setting."""
     return bool(lib().hj_set_dataflow(1 if on else 0))
+
+
+FACT_CHOLESKY, FACT_QR, FACT_QR_DRMAC = 0, 1, 2
+
+
+def set_factorization(f):
+    """Factorization of the block pairs: FACT_CHOLESKY (default), FACT_QR,
+    FACT_QR_DRMAC (the reference's build options USE_QR / USE_DRMAC);
+    returns the previous setting."""
+    return lib().hj_set_factorization(int(f))
 
 
 def last_dataflow():

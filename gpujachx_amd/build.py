@@ -49,7 +49,8 @@ def _hdf5_prefix():
 def build_library():
     os.makedirs(LIB, exist_ok=True)
     objs = []
-    for src in ["hj_kernels.hip", "hj_mlp.hip", "hj_batch.hip", "hj_host.cpp", "hj_strat.cpp"]:
+    for src in ["hj_kernels.hip", "hj_qr.hip", "hj_mlp.hip", "hj_batch.hip", "hj_host.cpp",
+                "hj_strat.cpp"]:
         obj = os.path.join(LIB, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
@@ -126,7 +127,8 @@ def build_pmc():
 def build_oracle(ref=True):
     out = os.path.join(ORACLE, "libhjoracle.so")
     _run(["gcc", "-O2", "-mfma", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC",
-          "-shared", "-Wall", "-o", out + ".tmp", os.path.join(ORACLE, "hjoracle.c"), "-lm"])
+          "-shared", "-Wall", "-o", out + ".tmp", os.path.join(ORACLE, "hjoracle.c"),
+          os.path.join(ORACLE, "hjoracle_qr.c"), "-lm"])
     os.replace(out + ".tmp", out)
     if ref and os.path.isdir("/root/reference/strat"):
         _run(["bash", os.path.join(ORACLE, "build_ref.sh"), "/root/reference"])

@@ -52,6 +52,9 @@ static float g_skip_min_propose = 0.1f;
 // converge in few sweeps; one sweep added costs more than skipping saves)
 static unsigned g_skip_min_visits = 32000u;
 // per-sweep visit records copied to the host (trace collection)
+// factorization of the block pairs: HJ_FACT_CHOLESKY (Gram + Cholesky),
+// HJ_FACT_QR, HJ_FACT_QR_DRMAC
+static int g_fact = HJ_FACT_CHOLESKY;
 // dataflow sweeps (one persistent launch per sweep) where the strategy allows
 static int g_flow = 0;
 static int g_last_flow = 0;  // the last run used them
@@ -399,6 +402,16 @@ static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, u
   const uint16_t *pairs = w.d_s1 + off * 2u;
   const uint8_t *skip = planned ? w.d_skip + off : nullptr;
   if (ev) HIP_CALL(hipEventRecord(ev[0], s));
+  if (g_fact != HJ_FACT_CHOLESKY) {
+    // QR path: the pair kernel reads the pair from G itself (no Gram)
+    if (ev) HIP_CALL(hipEventRecord(ev[1], s));
+    launch_pair_qr(s, sort, g_fact == HJ_FACT_QR_DRMAC, dG, lddG, nrow, pairs, w.np, w.d_W,
+                   w.d_flags, w.d_ctl, w.d_s0, w.s0steps, nswp, definite, nplus);
+    if (ev) HIP_CALL(hipEventRecord(ev[2], s));
+    launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
+    if (ev) HIP_CALL(hipEventRecord(ev[3], s));
+    return;
+  }
   launch_gram(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
   if (split)
     launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
@@ -428,6 +441,12 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
       --*steps_left;
     }
     ++issued;
+    if (g_fact != HJ_FACT_CHOLESKY) {
+      if (gram_launched) (*gram_launched)[st] = 0;
+      issue_step(s, w, st, sort, nswp, definite, nplus, dG, lddG, nrow, dV, lddV, ncol,
+                 evs ? evs + (size_t)4u * st : nullptr, false, false);
+      continue;
+    }
     if (gram_launched) (*gram_launched)[st] = have_gram ? 0 : 1;
     hipEvent_t *ev = evs ? evs + (size_t)4u * st : nullptr;
     const size_t off = (size_t)st * w.np;
@@ -512,15 +531,18 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
 
   Work w;
   w.alloc(nrow, ncol, full_svd);
-  const bool flow = g_flow && !full_svd && g_step_limit <= 0 && w.build_flow();
+  const bool flow = g_flow && !full_svd && g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY &&
+                    w.build_flow();
   g_last_flow = flow ? 1 : 0;
   if (full_svd)
     launch_initV(s, dV, 0u, ncol, lddV);
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
   double *const V = full_svd ? dV : nullptr;
-  const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready &&
+  // the forms act on the Gram of the Cholesky path; the QR path runs plain
+  const bool qr = g_fact != HJ_FACT_CHOLESKY;
+  const bool learned = !qr && (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready &&
                        (size_t)(ncol / 32u) * STRAT1_STEPS >= g_skip_min_visits;
-  const bool split = (g_form & HJ_FORM_SPLIT_GRAM) != 0u;
+  const bool split = !qr && (g_form & HJ_FORM_SPLIT_GRAM) != 0u;
   const bool planned = learned || split;
   if (learned)
     HIP_CALL(hipMemcpy(w.d_model, g_model_bf16, sizeof(uint16_t) * MLP_NPARAM, hipMemcpyHostToDevice));
@@ -925,6 +947,14 @@ extern "C" int hj_set_dataflow(int on)
 }
 
 extern "C" int hj_last_dataflow(void) { return g_last_flow; }
+
+extern "C" int hj_set_factorization(int f)
+{
+  const int prev = g_fact;
+  if (f == HJ_FACT_CHOLESKY || f == HJ_FACT_QR || f == HJ_FACT_QR_DRMAC)
+    g_fact = f;
+  return prev;
+}
 
 extern "C" unsigned hj_set_max_sweeps(unsigned n)
 {

@@ -125,6 +125,13 @@ struct FlowArgs {
   unsigned long long *trace;  // [nitems][3] clock at ticket, inputs ready, done; or null
 };
 
+// QR-path visit (hj_qr.hip): factorization of the pair read from G, level-0
+// sweeps (Drmac's for sorted definite blocks when drmac), W and the flags
+void launch_pair_qr(hipStream_t s, bool sort, bool drmac, const double *G, unsigned ldG,
+                    unsigned nrow, const uint16_t *pairs, unsigned np, double *W, unsigned *flags,
+                    RunCtl *ctl, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
+                    int definite, unsigned nplus);
+
 void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks);
 unsigned flow_blocks_per_cu();
 void launch_flow_reset(hipStream_t s, unsigned *cnt, size_t n);

@@ -267,6 +267,22 @@ void hj_set_step_limit(long steps);
 int hj_set_dataflow(int on);
 int hj_last_dataflow(void);
 
+/* Factorization of the block pairs, at run time where the reference chooses
+ * at build time (share/device_code_common.hpp:78-85,140-144):
+ * HJ_FACT_CHOLESKY -- Gram and Cholesky (the reference's default build,
+ *   share/device_code_common_Cholesky.hpp:4-171);
+ * HJ_FACT_QR -- R by 32-row panels, Householder QR of each and Givens merges
+ *   (-DUSE_QR: share/device_code_common_QR.hpp:4-94);
+ * HJ_FACT_QR_DRMAC -- the same with Drmac's overflow-safe sorted level-0
+ *   for sorted definite blocks (-DUSE_DRMAC:
+ *   share/device_code_cdsort_Drmac.hpp:4-316).
+ * The learned and split forms act on the Cholesky path only.  Returns the
+ * previous setting (an unknown value leaves it unchanged). */
+#define HJ_FACT_CHOLESKY 0
+#define HJ_FACT_QR       1
+#define HJ_FACT_QR_DRMAC 2
+int hj_set_factorization(int f);
+
 /* Device self-test of the scalar kernels: for n inputs, kind 0 = rsqrt(a),
  * 1 = trigonometric rotation (a=Apq,b=Dp,c=Dq), 2 = hyperbolic rotation.
  * Outputs c, t, big per input.  Returns 0 on success. */

@@ -54,6 +54,20 @@ def lib():
         L.hjo_set_max_sweeps.restype = u
         L.hjo_set_single_wave_gram.argtypes = [i]
         L.hjo_set_single_wave_gram.restype = i
+        L.hjo_set_factorization.argtypes = [i]
+        L.hjo_set_factorization.restype = i
+        L.hjo_nrm2_32.argtypes = [pd]
+        L.hjo_nrm2_32.restype = d
+        L.hjo_cosa.argtypes = [d, d, pd, pd]
+        L.hjo_cosa.restype = d
+        L.hjo_givens.argtypes = [d, d, pd, pd, pd]
+        L.hjo_givens.restype = None
+        L.hjo_qr32.argtypes = [pd]
+        L.hjo_qr32.restype = None
+        L.hjo_peeloff.argtypes = [pd, pd]
+        L.hjo_peeloff.restype = None
+        L.hjo_factorize_qr.argtypes = [pd, pd, u, u, pd]
+        L.hjo_factorize_qr.restype = None
         _L = L
     return _L
 
@@ -145,3 +159,61 @@ def pair(partials, sort, hyp, definite, npos, strat0, nswp=1):
                        s0.ctypes.data, s0.shape[0], nswp, _pd(W), ctypes.byref(b), ctypes.byref(upd),
                        _pd(R))
     return W.reshape(32, 32).T.copy(), R.reshape(32, 32).T.copy(), s, b.value, upd.value
+
+
+# factorization of the block pairs (the reference's build options)
+FACT_CHOLESKY, FACT_QR, FACT_QR_DRMAC = 0, 1, 2
+
+
+def set_factorization(f):
+    """Cholesky of the Gram (0), QR by 32-row panels (1), QR with Drmac's
+    sorted level-0 (2); returns the previous setting."""
+    return lib().hjo_set_factorization(int(f))
+
+
+def nrm2_32(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    assert x.shape == (32,)
+    return lib().hjo_nrm2_32(_pd(x))
+
+
+def cosa(ap, aq, x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().hjo_cosa(ap, aq, _pd(x), _pd(y))
+
+
+def givens(f, g):
+    c, s, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().hjo_givens(f, g, ctypes.byref(c), ctypes.byref(s), ctypes.byref(r))
+    return c.value, s.value, r.value
+
+
+def qr32(A):
+    """R of a 32x32 matrix by dQR32 (returns a new array)."""
+    F = np.asfortranarray(np.array(A, dtype=np.float64))
+    buf = F.T.ravel().copy()          # column-major
+    lib().hjo_qr32(_pd(buf))
+    return buf.reshape(32, 32).T.copy()
+
+
+def peeloff(R0, R1):
+    """R of [R0; R1] for upper triangular R0, R1 (dPeelOff); returns (R, R1')."""
+    b0 = np.asarray(R0, dtype=np.float64).T.ravel().copy()
+    b1 = np.asarray(R1, dtype=np.float64).T.ravel().copy()
+    lib().hjo_peeloff(_pd(b0), _pd(b1))
+    return b0.reshape(32, 32).T.copy(), b1.reshape(32, 32).T.copy()
+
+
+def factorize_qr(G):
+    """R (32x32) of an nrow x 32 column pair by 32-row panels."""
+    G = np.asarray(G, dtype=np.float64)
+    nrow = G.shape[0]
+    buf = G.T.ravel().copy()          # column-major, ld = nrow
+    R = np.zeros(1024)
+    L = lib()
+    base = buf.ctypes.data
+    L.hjo_factorize_qr(ctypes.cast(base, ctypes.POINTER(ctypes.c_double)),
+                       ctypes.cast(base + 16 * nrow * 8, ctypes.POINTER(ctypes.c_double)),
+                       nrow, nrow, _pd(R))
+    return R.reshape(32, 32).T.copy()

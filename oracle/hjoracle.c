@@ -240,7 +240,7 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
  * share/device_code_nosort.hpp:4-222 (sort == 0) and
  * share/device_code_cdsort_KISS.hpp:4-260 + share/device_code_cdsort.hpp:25-167
  * (sort == 1).  hyp: the block pair mixes J signs (local npos). */
-static void l0(double *R, double *W, int sort, int hyp, int definite,
+void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
                unsigned npos, const uint8_t *strat0, unsigned s0steps,
                unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out)
 {
@@ -392,7 +392,7 @@ unsigned hjo_pair(const double *partials, unsigned nchunk, int sort,
     for (unsigned x = 0u; x < 32u; ++x)
       F32(W, x, y) = (x == y) ? 1.0 : 0.0;
   unsigned s = 0u, b = 0u;
-  l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b);
+  hjo_l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b);
   if (Rout)
     memcpy(Rout, R, sizeof(R));
   *big = b;
@@ -458,6 +458,17 @@ static void local_npos(unsigned nplus, unsigned iBlk, unsigned jBlk,
 
 static unsigned g_max_sweeps = HJO_NSWEEP;
 static int g_single_wave_gram = 0;
+static int g_fact = HJO_FACT_CHOLESKY;
+
+/* the factorization of the block pairs (the reference's build options
+ * USE_QR / USE_DRMAC, share/device_code_common.hpp:78-85,140-144) */
+int hjo_set_factorization(int f)
+{
+  const int old = g_fact;
+  if (f >= HJO_FACT_CHOLESKY && f <= HJO_FACT_QR_DRMAC)
+    g_fact = f;
+  return old;
+}
 
 /* Gram order of the batched launch: one chunk, one wave over all rows. */
 int hjo_set_single_wave_gram(int on)
@@ -550,11 +561,6 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
         double *const G0 = G + (size_t)iBlk * 16u * ldG;
         double *const G1 = G + (size_t)jBlk * 16u * ldG;
         double *const pp = part + (size_t)pi * 768u * nchunk;
-        if (g_single_wave_gram)
-          gram_wave(G0, G1, ldG, 0u, nrow, pp);
-        else
-          for (unsigned c = 0u; c < nchunk; ++c)
-            hjo_gram_partial(G0, G1, ldG, c * CH, CH, pp + (size_t)c * 768u);
         int hyp = 0, def = definite;
         unsigned npos = 0u;
         if (!definite) {
@@ -562,10 +568,21 @@ int hjo_hypJacL2(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
           hyp = (def == 0);
         }
         double W[1024];
-        unsigned b = 0u;
+        unsigned b = 0u, s = 0u;
         int upd = 0;
-        const unsigned s = hjo_pair(pp, nchunk, cdsort, hyp, def, npos, strat0,
-                                    s0steps, nswp, W, &b, &upd, NULL);
+        if (g_fact != HJO_FACT_CHOLESKY) {
+          s = hjo_pair_qr(G0, G1, ldG, nrow, cdsort, hyp, def, npos,
+                          g_fact == HJO_FACT_QR_DRMAC, strat0, s0steps, nswp, W, &b, &upd);
+        }
+        else {
+          if (g_single_wave_gram)
+            gram_wave(G0, G1, ldG, 0u, nrow, pp);
+          else
+            for (unsigned c = 0u; c < nchunk; ++c)
+              hjo_gram_partial(G0, G1, ldG, c * CH, CH, pp + (size_t)c * 768u);
+          s = hjo_pair(pp, nchunk, cdsort, hyp, def, npos, strat0,
+                       s0steps, nswp, W, &b, &upd, NULL);
+        }
         ps[2u * pi] = s;
         ps[2u * pi + 1u] = b;
         if (upd) {

@@ -88,6 +88,35 @@ unsigned hjo_pair(const double *partials, unsigned nchunk, int sort,
                   const uint8_t *strat0, unsigned s0steps, unsigned nswp,
                   double W[1024], unsigned *big, int *upd, double R[1024]);
 
+/* level-0 sweeps on R (32x32 upper triangular, column-major) accumulating
+ * W (identity on entry); returns the rotation counts. */
+void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
+            unsigned npos, const uint8_t *strat0, unsigned s0steps,
+            unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out);
+
+/* Factorization of the block pairs: Cholesky of the Gram (the reference's
+ * default build), QR by 32-row panels (USE_QR), QR with Drmac's sorted
+ * level-0 (USE_DRMAC).  Returns the previous setting. */
+#define HJO_FACT_CHOLESKY 0
+#define HJO_FACT_QR       1
+#define HJO_FACT_QR_DRMAC 2
+int hjo_set_factorization(int f);
+/* the QR path's pieces (hjoracle_qr.c) */
+double hjo_nrm2_32(const double x[32]);
+double hjo_cosa(double Ap, double Aq, const double x[32], const double y[32]);
+void hjo_givens(double f, double g, double *c, double *s, double *r);
+void hjo_qr32(double A[1024]);
+void hjo_peeloff(double R0[1024], double R1[1024]);
+void hjo_factorize_qr(const double *G0, const double *G1, unsigned ld, unsigned nrow,
+                      double R[1024]);
+void hjo_l0_drmac(double *R, double *W, int definite, const uint8_t *strat0,
+                  unsigned s0steps, unsigned nswp, unsigned *blk_s_out,
+                  unsigned *blk_b_out);
+unsigned hjo_pair_qr(const double *G0, const double *G1, unsigned ld, unsigned nrow,
+                     int sort, int hyp, int definite, unsigned npos, int drmac,
+                     const uint8_t *strat0, unsigned s0steps, unsigned nswp,
+                     double W[1024], unsigned *big, int *upd);
+
 /* [G0 G1](rows 0..nr-1) <- [G0 G1] * W  (device order) */
 void hjo_update(double *G0, double *G1, unsigned ld, unsigned nr,
                 const double W[1024]);
