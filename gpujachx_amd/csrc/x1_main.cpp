@@ -8,6 +8,8 @@
 // tables SNP0 (order 32) and SNP1 (order ceil(H5G/16)) from the plugin SDY,
 // prints the per-sweep lines and the totals, and, if H5R is given, writes a
 // new group H5G there with IDADIM, ISTATS, TIMING, G, V (if ALG & 2) and D.
+// The reference's build options USE_QR / USE_DRMAC are chosen at run time by
+// the environment variable HJ_FACT = qr | drmac (default: Cholesky).
 #include <hdf5.h>
 #include <hdf5_hl.h>
 
@@ -52,6 +54,15 @@ int main(int argc, char *argv[])
   const unsigned n0 = HYPJACL1_NCOLB << 1u;
   const unsigned n1 = (static_cast<unsigned>(atoi(ca_h5g)) + HYPJACL1_NCOLB - 1u) / HYPJACL1_NCOLB;
   init_strats(ca_sdy, ca_snp0, n0, ca_snp1, n1);
+
+  if (const char *f = getenv("HJ_FACT")) {
+    if (!strcmp(f, "qr"))
+      (void)hj_set_factorization(HJ_FACT_QR);
+    else if (!strcmp(f, "drmac"))
+      (void)hj_set_factorization(HJ_FACT_QR_DRMAC);
+    else if (strcmp(f, "cholesky"))
+      X1_DIE("HJ_FACT must be qr, drmac or cholesky");
+  }
 
   const int dev = atoi(ca_dev);
   const int dcc = configureGPU(dev);

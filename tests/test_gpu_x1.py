@@ -58,3 +58,30 @@ def test_x1_argument_error_exit_code(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert "error -2" in p.stderr
     assert p.returncode == (256 - 2) % 256
+
+
+@pytest.mark.skipif(not have_tools, reason="HDF5 tools not built")
+@pytest.mark.parametrize("fact,ofact", [("qr", o.FACT_QR), ("drmac", o.FACT_QR_DRMAC)])
+def test_x1_factorization_from_environment(tmp_path, fact, ofact):
+    # HJ_FACT selects the reference's USE_QR / USE_DRMAC variants at run time
+    m = n = 128
+    s = 10.0 ** (-8.0 * np.arange(n) / (n - 1))
+    G = svd_input(m, n, s, seed=2)
+    fin = stage(str(tmp_path), "in", G, n, lam=s ** 2)
+    env = dict(os.environ, HJ_QUIET="0", HJ_FACT=fact)
+    p = subprocess.run([X1, "0", g.STRAT_SO, "cycwor", "cycwor", "9", fin, str(n)],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert p.returncode == 0, p.stderr
+    glb = re.search(r"GLB_ROT_S\(\s*(\d+)\), GLB_ROT_B\(\s*(\d+)\)", p.stdout)
+    g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+    s0, s1 = g.current_strats()
+    old = o.set_factorization(ofact)
+    try:
+        ro = o.hypJacL2(9, G, n, s0, s1)
+    finally:
+        o.set_factorization(old)
+    assert (int(glb.group(1)), int(glb.group(2))) == (ro["rot_s"], ro["rot_b"])
+    bad = subprocess.run([X1, "0", g.STRAT_SO, "cycwor", "cycwor", "9", fin, str(n)],
+                         capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, HJ_FACT="lu"))
+    assert bad.returncode != 0 and "HJ_FACT" in bad.stderr
