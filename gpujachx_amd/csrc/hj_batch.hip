@@ -24,6 +24,8 @@
 
 #include "../../include/hypjac.h"
 #include "hj_device.hpp"
+#include "hj_fused.hpp"
+#include "hj_kernels.hpp"
 #include "hj_strat.hpp"
 #include "hj_util.hpp"
 
@@ -33,7 +35,10 @@ namespace hj {
 namespace {
 
 constexpr unsigned BLDR = L0LD;                       // padded leading dimension (the level-0 layout)
-constexpr unsigned WAVE_LDS = 2u * 32u * BLDR;        // doubles of R and W per wave
+// doubles per wave: R and W (the fused pass: the W of both pairs, LDW rows),
+// and the 16x16 transpose buffer of the fused pass's stores
+constexpr unsigned WAVE_LDS = 2u * 32u * BLDR + 16u * LDT;
+static_assert(BLDR == LDW, "the fused pass keeps its W copies in the R / W regions");
 constexpr unsigned BATCH_MAX_WAVES = 8u;
 // waves per SIMD the batch kernel is compiled for (register budget 512/n)
 #ifndef BATCH_WAVES_PER_SIMD
@@ -45,8 +50,9 @@ struct BProb {
   unsigned nrow, ncol, nplus, ldG, ldV, routine;
   unsigned s1off;  // first pair of this problem's strategy table in the pool
   unsigned steps, np;
-  unsigned pad;
+  unsigned qoff;  // first closed quad of this problem's table in the quad pool, or NO_QUADS
 };
+constexpr unsigned NO_QUADS = 0xFFFFFFFFu;
 
 struct BOut {
   unsigned sweeps, pad;
@@ -99,13 +105,14 @@ __device__ void wave_gram(const double *__restrict__ G, unsigned ldG, unsigned n
 // [M_i M_j] <- [M_i M_j] W over rows [0, nrow) by one wave, 16-row slabs
 // (the arithmetic of k_update: per element an fma chain over k = 0..31).
 __device__ void wave_update(double *__restrict__ M, unsigned ld, unsigned nrow, unsigned iBlk,
-                            unsigned jBlk, const double *W, unsigned l)
+                            unsigned jBlk, const double *W, unsigned l, unsigned wld)
 {
+  // W(k, j) at W[j * wld + k] (the LDS copy, wld = BLDR, or the scratch, 32)
   double a0[8], a1[8];
 #pragma unroll
   for (unsigned u = 0u; u < 8u; ++u) {
-    a0[u] = BR(W, 4u * u + (l >> 4), (l & 15u));
-    a1[u] = BR(W, 4u * u + (l >> 4), 16u + (l & 15u));
+    a0[u] = W[(l & 15u) * wld + 4u * u + (l >> 4)];
+    a1[u] = W[(16u + (l & 15u)) * wld + 4u * u + (l >> 4)];
   }
   double *cp[8];
 #pragma unroll
@@ -149,12 +156,13 @@ __device__ void wave_update(double *__restrict__ M, unsigned ld, unsigned nrow, 
   }
 }
 
-// One block pair by one wave: Gram, Cholesky, level-0 sweep(s), update.
-// Returns the rotation counts of the pair (s, b) in every lane.
+// The visit of one block pair by one wave on the Gram in R (BR layout, the
+// lower triangle): Cholesky, level-0 sweep(s), W in W.  Returns the rotation
+// counts of the pair (s, b) in every lane.
 template <bool SORT>
-__device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
-                          const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
-                          unsigned l, unsigned &out_s, unsigned &out_b)
+__device__ void wave_visit(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
+                           const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
+                           unsigned l, unsigned &out_s, unsigned &out_b)
 {
   // local J structure, share/device_code_nosort_accumV.hpp:53-76
   bool hyp = false;
@@ -169,11 +177,6 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
     else { npos = 32u; def = +1; }
     hyp = (def == 0);
   }
-
-  wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   // Cholesky, share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT)
   {
@@ -209,10 +212,83 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
   wave_level0<SORT, false>(R, W, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, cos_unused);
   out_s = blk_s;
   out_b = blk_b;
-  if (SORT || blk_s) {
-    wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l);
+}
+
+// One block pair by one wave: Gram, visit, update (the unfused step).
+template <bool SORT>
+__device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
+                          const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
+                          unsigned l, unsigned &out_s, unsigned &out_b)
+{
+  wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
+  wave_lds_sync();
+  wave_visit<SORT>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, out_s, out_b);
+  if (SORT || out_s) {
+    wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l, BLDR);
     if (P.routine & HYPJAC_FULL_SVD)
-      wave_update(P.V, P.ldV, P.ncol, iBlk, jBlk, W, l);
+      wave_update(P.V, P.ldV, P.ncol, iBlk, jBlk, W, l, BLDR);
+  }
+}
+
+// Gram tiles of a pair (00, 10, 11; lane l, register r at l*4 + r, the MFMA
+// output layout) from the fused pass's scratch into R, as wave_gram leaves
+// them
+__device__ __forceinline__ void gram_to_R(const double *__restrict__ g, double *R, unsigned l)
+{
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    const unsigned i = 4u * r + (l >> 4), j = l & 15u;
+    BR(R, i, j) = g[l * 4u + r];
+    BR(R, 16u + i, j) = g[256u + l * 4u + r];
+    BR(R, 16u + i, 16u + j) = g[512u + l * 4u + r];
+  }
+}
+
+// The fused transition of a closed quad by one wave over all rows: the
+// update of both pairs of step s (W0, W1 in LDS, W(k, j) at k*LDW + j) and
+// the Gram of both pairs of step s+1 from the updated values in registers,
+// in the single-wave Gram's order (16-row slabs ascending), to g0, g1.
+__device__ void wave_upgram(double *__restrict__ G, unsigned ldG, unsigned nrow, unsigned A0,
+                            unsigned B0, unsigned A1, unsigned B1, bool u0, bool u1, const Quad &Q,
+                            const double *W0, const double *W1, double *T, unsigned l,
+                            double *__restrict__ g0, double *__restrict__ g1)
+{
+  d4 z = {0.0, 0.0, 0.0, 0.0};
+  d4 g00 = z, g10 = z, g11 = z, h00 = z, h10 = z, h11 = z;
+  double xa[8], xb[8];
+  slab_load(G, ldG, 0u, A0, B0, u0, l, xa);
+  slab_load(G, ldG, 0u, A1, B1, u1, l, xb);
+  for (unsigned r = 0u; r < nrow; r += 16u) {
+    double ca[8], cb[8];
+#pragma unroll
+    for (unsigned u = 0u; u < 8u; ++u) {
+      ca[u] = xa[u];
+      cb[u] = xb[u];
+    }
+    if (r + 16u < nrow) {
+      slab_load(G, ldG, r + 16u, A0, B0, u0, l, xa);
+      slab_load(G, ldG, r + 16u, A1, B1, u1, l, xb);
+    }
+    d4 t0, t1, t2, t3;
+    slab_compute(G, ldG, r, A0, B0, u0, W0, l, ca, t0, t1, T);
+    slab_compute(G, ldG, r, A1, B1, u1, W1, l, cb, t2, t3, T);
+    {
+      const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
+      gram_fp64_slab(X, Y, g00, g10, g11);
+    }
+    {
+      const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
+      gram_fp64_slab(X, Y, h00, h10, h11);
+    }
+  }
+#pragma unroll
+  for (unsigned r = 0u; r < 4u; ++r) {
+    g0[l * 4u + r] = g00[r];
+    g0[256u + l * 4u + r] = g10[r];
+    g0[512u + l * 4u + r] = g11[r];
+    g1[l * 4u + r] = h00[r];
+    g1[256u + l * 4u + r] = h10[r];
+    g1[512u + l * 4u + r] = h11[r];
   }
 }
 
@@ -226,7 +302,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BATCH_WAVES
                                                unsigned *__restrict__ queue,
                                                const uint16_t *__restrict__ s1pool,
                                                const uint8_t *__restrict__ strat0, unsigned s0steps,
-                                               unsigned max_sweeps, BOut *__restrict__ out)
+                                               unsigned max_sweeps, BOut *__restrict__ out,
+                                               const Quad *__restrict__ qpool,
+                                               double *__restrict__ scr,
+                                               unsigned *__restrict__ sflags, unsigned np_max)
 {
   extern __shared__ double lds[];
   __shared__ uint8_t S0[1024];
@@ -235,6 +314,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BATCH_WAVES
   const unsigned l = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double *const R = lds + (size_t)wv * WAVE_LDS;
   double *const W = R + 32u * BLDR;
+  double *const T = W + 32u * BLDR;
+  // this workgroup's scratch of the fused steps: the Grams of the next
+  // step's pairs, the W and update flags of the running step's pairs
+  double *const sg = scr + (size_t)blockIdx.x * np_max * (768u + 1024u);
+  double *const sw = sg + (size_t)np_max * 768u;
+  unsigned *const sf = sflags + (size_t)blockIdx.x * np_max;
   for (unsigned e = threadIdx.x; e < s0steps * 32u; e += blockDim.x)
     S0[e] = strat0[e];
   for (;;) {
@@ -269,15 +354,88 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BATCH_WAVES
       unsigned sw_s = 0u, sw_b = 0u;
       for (unsigned st = 0u; st < P.steps; ++st) {
         __syncthreads();
+        const bool fused = (P.qoff != NO_QUADS) && (st + 1u < P.steps);
+        if (!fused && (P.qoff == NO_QUADS || st == 0u)) {
+          // unfused step: each wave's pairs, Gram -> visit -> update
+          for (unsigned pi = wv; pi < P.np; pi += nw) {
+            const unsigned iBlk = tab[2u * (st * P.np + pi)], jBlk = tab[2u * (st * P.np + pi) + 1u];
+            unsigned s = 0u, b = 0u;
+            if (sort)
+              wave_pair<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            else
+              wave_pair<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            sw_s += s;
+            sw_b += b;
+          }
+          continue;
+        }
+        // fused steps: the visits (Gram of step 0 from G, later ones from
+        // the previous transition's pass), then one pass per closed quad
+        // updating both pairs and forming the next step's Grams; the last
+        // step of a sweep updates after each visit
         for (unsigned pi = wv; pi < P.np; pi += nw) {
           const unsigned iBlk = tab[2u * (st * P.np + pi)], jBlk = tab[2u * (st * P.np + pi) + 1u];
+          if (st == 0u)
+            wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
+          else
+            gram_to_R(sg + (size_t)pi * 768u, R, l);
+          wave_lds_sync();
           unsigned s = 0u, b = 0u;
           if (sort)
-            wave_pair<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            wave_visit<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
           else
-            wave_pair<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            wave_visit<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
           sw_s += s;
           sw_b += b;
+          const bool upd = sort || s;
+          if (!fused) {
+            if (upd) {
+              wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l, BLDR);
+              if (full)
+                wave_update(P.V, P.ldV, P.ncol, iBlk, jBlk, W, l, BLDR);
+            }
+          }
+          else {
+            double *const w = sw + (size_t)pi * 1024u;
+#pragma unroll
+            for (unsigned e = 0u; e < 16u; ++e) {
+              const unsigned k = l + 64u * e;
+              w[k] = BR(W, k & 31u, k >> 5);
+            }
+            if (l == 0u)
+              sf[pi] = upd ? 1u : 0u;
+          }
+          wave_lds_sync();
+        }
+        if (!fused)
+          continue;
+        __syncthreads();
+        const unsigned nq = P.np / 2u;
+        const Quad *const Qs = qpool + P.qoff + (size_t)st * nq;
+        for (unsigned qi = wv; qi < nq; qi += nw) {
+          const Quad Q = Qs[qi];
+          const unsigned A0 = tab[2u * (st * P.np + Q.p0)], B0 = tab[2u * (st * P.np + Q.p0) + 1u];
+          const unsigned A1 = tab[2u * (st * P.np + Q.p1)], B1 = tab[2u * (st * P.np + Q.p1) + 1u];
+          const bool u0 = sf[Q.p0] != 0u, u1 = sf[Q.p1] != 0u;
+          const double *const w0 = sw + (size_t)Q.p0 * 1024u;
+          const double *const w1 = sw + (size_t)Q.p1 * 1024u;
+          // W(k, j) of both pairs into this wave's R / W regions (k * LDW + j)
+#pragma unroll
+          for (unsigned e = 0u; e < 16u; ++e) {
+            const unsigned k = l + 64u * e;
+            R[(k & 31u) * LDW + (k >> 5)] = w0[k];
+            W[(k & 31u) * LDW + (k >> 5)] = w1[k];
+          }
+          wave_lds_sync();
+          wave_upgram(P.G, P.ldG, P.nrow, A0, B0, A1, B1, u0, u1, Q, R, W, T, l,
+                      sg + (size_t)Q.n0 * 768u, sg + (size_t)Q.n1 * 768u);
+          if (full) {
+            if (u0)
+              wave_update(P.V, P.ldV, P.ncol, A0, B0, w0, l, 32u);
+            if (u1)
+              wave_update(P.V, P.ldV, P.ncol, A1, B1, w1, l, 32u);
+          }
+          wave_lds_sync();
         }
       }
       if (l == 0u) {
@@ -347,6 +505,10 @@ struct Plan {
   BProb *d_probs = nullptr;
   unsigned *d_order = nullptr, *d_queue = nullptr;
   uint16_t *d_pool = nullptr;
+  Quad *d_quads = nullptr;       // closed quads of the fusable tables
+  double *d_scr = nullptr;       // per workgroup: Grams and W of the fused steps
+  unsigned *d_sflags = nullptr;  // per workgroup: update flags of the fused steps
+  unsigned np_max = 1u;
   uint8_t *d_s0 = nullptr;
   unsigned s0steps = 0u;
   BOut *d_out = nullptr;
@@ -419,8 +581,10 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
   P->cfg = cfg;
   // strategy tables: one per distinct (family, n1)
   std::vector<uint16_t> pool;
+  std::vector<Quad> qpool;
   std::vector<std::pair<std::string, unsigned>> keys;
-  std::vector<unsigned> key_off, key_steps;
+  std::vector<unsigned> key_off, key_steps, key_qoff;
+  const bool nofuse = getenv("HJ_BATCH_NOFUSE") != nullptr;
   P->h_probs.resize(n);
   for (unsigned i = 0u; i < n; ++i) {
     const hj_batch_desc &d = probs[i];
@@ -437,6 +601,15 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
       keys.push_back({fam, n1});
       key_off.push_back((unsigned)(pool.size() / 2u));
       key_steps.push_back(t.steps);
+      // fused steps when every transition of the table has closed quads
+      std::vector<uint16_t> flat(t.tab.begin(), t.tab.end());
+      std::vector<Quad> q;
+      std::vector<char> closed;
+      bool all = !nofuse && closed_quads(flat.data(), t.steps, t.pairs, q, &closed);
+      for (unsigned st = 0u; all && st + 1u < t.steps; ++st)
+        all = closed[st] != 0;
+      key_qoff.push_back(all ? (unsigned)qpool.size() : NO_QUADS);
+      if (all) qpool.insert(qpool.end(), q.begin(), q.end());
       pool.insert(pool.end(), t.tab.begin(), t.tab.end());
     }
     BProb b;
@@ -452,7 +625,8 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
     b.s1off = key_off[k];
     b.steps = key_steps[k];
     b.np = n1 / 2u;
-    b.pad = 0u;
+    b.qoff = key_qoff[k];
+    P->np_max = std::max(P->np_max, b.np);
     P->h_probs[i] = b;
   }
   std::vector<unsigned> order(n);
@@ -477,6 +651,9 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
   HIP_CALL(hipMemcpy(P->d_order, order.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
   HIP_CALL(hipMemcpy(P->d_pool, pool.data(), sizeof(uint16_t) * pool.size(), hipMemcpyHostToDevice));
   HIP_CALL(hipMemcpy(P->d_s0, s0b.data(), 1024u, hipMemcpyHostToDevice));
+  HIP_CALL(hipMalloc(&P->d_quads, sizeof(Quad) * std::max<size_t>(qpool.size(), 1u)));
+  if (!qpool.empty())
+    HIP_CALL(hipMemcpy(P->d_quads, qpool.data(), sizeof(Quad) * qpool.size(), hipMemcpyHostToDevice));
   P->h_out.resize(n);
 
   int dev = 0;
@@ -495,6 +672,8 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
   }
   HIP_CALL(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds));
+  HIP_CALL(hipMalloc(&P->d_scr, sizeof(double) * (768u + 1024u) * P->np_max * P->grid));
+  HIP_CALL(hipMalloc(&P->d_sflags, sizeof(unsigned) * P->np_max * P->grid));
   int rate_khz = 0;
   if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && rate_khz > 0)
     P->clock_mhz = rate_khz / 1000.0;
@@ -507,7 +686,7 @@ extern "C" void *hj_batch_plan(const hj_batch_desc *probs, unsigned n, const hj_
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, P->cap, P->d_queue);
   hipLaunchKernelGGL(k_batch, dim3(P->grid), dim3(64u * cfg.waves), P->lds, P->cap, P->d_probs,
                      P->d_order, n, P->d_queue, P->d_pool, P->d_s0, P->s0steps, cfg.max_sweeps,
-                     P->d_out);
+                     P->d_out, P->d_quads, P->d_scr, P->d_sflags, P->np_max);
   HIP_CALL(hipStreamEndCapture(P->cap, &graph));
   HIP_CALL(hipGraphInstantiate(&P->gexec, graph, nullptr, nullptr, 0));
   HIP_CALL(hipGraphDestroy(graph));
@@ -558,7 +737,8 @@ extern "C" void hj_batch_free(void *plan)
   if (P->gexec) HIP_CALL(hipGraphExecDestroy(P->gexec));
   if (P->cap) HIP_CALL(hipStreamDestroy(P->cap));
   for (void *p : {(void *)P->d_probs, (void *)P->d_order, (void *)P->d_queue, (void *)P->d_pool,
-                  (void *)P->d_s0, (void *)P->d_out})
+                  (void *)P->d_s0, (void *)P->d_out, (void *)P->d_quads, (void *)P->d_scr,
+                  (void *)P->d_sflags})
     if (p) HIP_CALL(hipFree(p));
   delete P;
 }

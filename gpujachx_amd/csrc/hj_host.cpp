@@ -89,6 +89,65 @@ static unsigned update_rows(unsigned nrow, unsigned nV, unsigned np)
   return ch;
 }
 
+// The closed quads of every transition t -> t+1 of a level-1 table (flat
+// [steps][np][2]): for each pair of step t, the pair holding its first
+// block's next partner; the transition is closed when the two pairs' four
+// blocks form two pairs of step t+1.  out: [steps-1][np/2] (entries of
+// open transitions undefined); fused[t] = transition t is closed.  Returns
+// whether any transition is closed.
+bool closed_quads(const uint16_t *tab, unsigned steps, unsigned np, std::vector<Quad> &out,
+                  std::vector<char> *fused)
+{
+  if (fused) fused->assign(steps, 0);
+  if (np < 2u || (np & 1u) || steps < 2u) return false;
+  out.assign((size_t)(steps - 1u) * (np / 2u), Quad());
+  const unsigned nb = 2u * np;
+  std::vector<int> slotA(nb), slotB(nb), partB(nb);
+  bool any = false;
+  auto T = [&](unsigned t, unsigned p, unsigned k) { return (unsigned)tab[((size_t)t * np + p) * 2u + k]; };
+  for (unsigned t = 0u; t + 1u < steps; ++t) {
+    for (unsigned p = 0u; p < np; ++p) {
+      slotA[T(t, p, 0)] = (int)p;
+      slotA[T(t, p, 1)] = (int)p;
+      const unsigned x = T(t + 1u, p, 0), y = T(t + 1u, p, 1);
+      slotB[x] = (int)p;
+      slotB[y] = (int)p;
+      partB[x] = (int)y;
+      partB[y] = (int)x;
+    }
+    std::vector<char> done(np, 0);
+    unsigned nq = 0u;
+    bool ok = true;
+    for (unsigned p = 0u; p < np && ok; ++p) {
+      if (done[p]) continue;
+      const unsigned a = T(t, p, 0), b = T(t, p, 1);
+      const unsigned p1 = (unsigned)slotA[partB[a]];
+      if (p1 == p || done[p1]) { ok = false; break; }
+      const unsigned c = T(t, p1, 0), d = T(t, p1, 1);
+      const unsigned pb = (unsigned)partB[b];
+      if (!((pb == c || pb == d) && pb != (unsigned)partB[a])) { ok = false; break; }
+      auto tile = [&](unsigned blk) -> uint8_t {
+        return (uint8_t)((blk == a) ? 0u : (blk == b) ? 1u : (blk == c) ? 2u : 3u);
+      };
+      Quad q;
+      q.p0 = (uint16_t)p;
+      q.p1 = (uint16_t)p1;
+      q.n0 = (uint16_t)slotB[a];
+      q.n1 = (uint16_t)slotB[b];
+      q.x0 = tile(T(t + 1u, q.n0, 0));
+      q.y0 = tile(T(t + 1u, q.n0, 1));
+      q.x1 = tile(T(t + 1u, q.n1, 0));
+      q.y1 = tile(T(t + 1u, q.n1, 1));
+      done[p] = done[p1] = 1;
+      out[(size_t)t * (np / 2u) + nq++] = q;
+    }
+    const bool closed = ok && nq == np / 2u;
+    if (fused) (*fused)[t] = closed ? 1 : 0;
+    any = any || closed;
+  }
+  return any;
+}
+
 // Device workspace of one level-2 run.
 struct Work {
   unsigned np = 0u, steps = 0u, s0steps = 0u, nchunk = 0u, CH = 0u, chG = 0u, chV = 0u;
@@ -169,48 +228,15 @@ struct Work {
   void build_quads()
   {
     fuse.assign(steps, 0);
-    if (np < 2u || (np & 1u) || steps < 2u || getenv("HJ_NOFUSE")) return;
-    std::vector<Quad> all((size_t)(steps - 1u) * (np / 2u));
-    const unsigned nb = 2u * np;
-    std::vector<int> slotA(nb), slotB(nb), partB(nb);
-    for (unsigned t = 0u; t + 1u < steps; ++t) {
+    if (getenv("HJ_NOFUSE")) return;
+    std::vector<uint16_t> tab((size_t)steps * np * 2u);
+    for (unsigned t = 0u; t < steps; ++t)
       for (unsigned p = 0u; p < np; ++p) {
-        slotA[strat1[t][p][0]] = (int)p;
-        slotA[strat1[t][p][1]] = (int)p;
-        const unsigned x = strat1[t + 1u][p][0], y = strat1[t + 1u][p][1];
-        slotB[x] = (int)p;
-        slotB[y] = (int)p;
-        partB[x] = (int)y;
-        partB[y] = (int)x;
+        tab[((size_t)t * np + p) * 2u] = strat1[t][p][0];
+        tab[((size_t)t * np + p) * 2u + 1u] = strat1[t][p][1];
       }
-      std::vector<char> done(np, 0);
-      unsigned nq = 0u;
-      bool ok = true;
-      for (unsigned p = 0u; p < np && ok; ++p) {
-        if (done[p]) continue;
-        const unsigned a = strat1[t][p][0], b = strat1[t][p][1];
-        const unsigned p1 = (unsigned)slotA[partB[a]];
-        if (p1 == p || done[p1]) { ok = false; break; }
-        const unsigned c = strat1[t][p1][0], d = strat1[t][p1][1];
-        const unsigned pb = (unsigned)partB[b];
-        if (!((pb == c || pb == d) && pb != (unsigned)partB[a])) { ok = false; break; }
-        auto tile = [&](unsigned blk) -> uint8_t {
-          return (uint8_t)((blk == a) ? 0u : (blk == b) ? 1u : (blk == c) ? 2u : 3u);
-        };
-        Quad q;
-        q.p0 = (uint16_t)p;
-        q.p1 = (uint16_t)p1;
-        q.n0 = (uint16_t)slotB[a];
-        q.n1 = (uint16_t)slotB[b];
-        q.x0 = tile(strat1[t + 1u][q.n0][0]);
-        q.y0 = tile(strat1[t + 1u][q.n0][1]);
-        q.x1 = tile(strat1[t + 1u][q.n1][0]);
-        q.y1 = tile(strat1[t + 1u][q.n1][1]);
-        done[p] = done[p1] = 1;
-        all[(size_t)t * (np / 2u) + nq++] = q;
-      }
-      fuse[t] = (ok && nq == np / 2u) ? 1 : 0;
-    }
+    std::vector<Quad> all;
+    if (!closed_quads(tab.data(), steps, np, all, &fuse)) return;
     HIP_CALL(hipMalloc(&d_quads, sizeof(Quad) * all.size()));
     HIP_CALL(hipMemcpy(d_quads, all.data(), sizeof(Quad) * all.size(), hipMemcpyHostToDevice));
     hquads.swap(all);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #define STRAT0_TAB_BYTES 1024u  // 32 steps x 16 pairs x 2 indices
 #define MAX_GRAM_CHUNKS 16u     // upper bound of gram_nchunk()
@@ -80,6 +81,9 @@ struct Quad {
   uint16_t n0, n1;          // slots of the next pairs in step s+1
   uint8_t x0, y0, x1, y1;   // tiles of the first / second block of n0 and n1
 };
+
+bool closed_quads(const uint16_t *tab, unsigned steps, unsigned np, std::vector<Quad> &out,
+                  std::vector<char> *fused);
 
 void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
                    const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
