@@ -267,7 +267,7 @@ def main():
     ap.add_argument("--out", default=DEFAULT_MODEL)
     ap.add_argument("--max-samples", type=int, default=3_000_000)
     ap.add_argument("--epochs", type=int, default=6)
-    ap.add_argument("--threshold", type=float, default=0.2)
+    ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--max-skips", type=int, default=8)
     ap.add_argument("--max-n", type=int, default=8192, help="largest training order")
     ap.add_argument("--sizes", type=int, nargs="*", help="eval: orders to evaluate")
