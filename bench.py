@@ -113,6 +113,8 @@ def main():
                     help="skip the unprofiled (graph-launched) timing of the same workload")
     ap.add_argument("--flow", choices=["on", "off"], default="off",
                     help="dataflow sweeps (one persistent launch per sweep) for both forms")
+    ap.add_argument("--staged", choices=["on", "off"], default="on",
+                    help="staged sweeps (pair visits overlapped with half-step block updates)")
     ap.add_argument("--model", default=None, help="skip-model file (default: the packaged one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dgesvj", action="store_true", help="also time LAPACK dgesvj at N=1024")
@@ -131,6 +133,7 @@ def main():
     g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
     g.set_max_sweeps(args.max_sweeps)
     g.set_dataflow(args.flow == "on")
+    g.set_staged(args.staged == "on")
     L = g.lib()
     G0, s_exact = make_input(torch, n, args.seed)
     W = torch.empty_like(G0)
@@ -169,13 +172,13 @@ def main():
         for _ in range(warmup):
             one()
         g.set_profile(profile)
-        kms, kcnt = np.zeros(4), np.zeros(4)
+        kms, kcnt = np.zeros(5), np.zeros(5)
         visits = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             one()
-            for k in range(4):
+            for k in range(5):
                 ms, c = g.kernel_ms(k)
                 kms[k] += ms
                 kcnt[k] += c
@@ -188,7 +191,7 @@ def main():
         splits = g.last_splits()
         sig = torch.sqrt(torch.abs(D)).cpu().numpy()
         return {"per": elapsed / steps, "kms": kms, "kcnt": kcnt, "visits": visits,
-                "flow": g.last_dataflow(),
+                "flow": g.last_dataflow(), "staged": g.last_staged(),
                 "sweeps": sw.value, "rot_b": rot_b, "skipped": skips[1], "split": splits[1],
                 "split_per_sweep": splits[0], "skip_per_sweep": skips[0], "sigma": sig}
 
@@ -251,6 +254,18 @@ def main():
         upd_ms = kms[3] / fl_n
         ach = upd_bytes / (upd_ms * 1e-3) / 1e9
         upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
+    staged_used = bool(main_run["staged"])
+    if staged_used:
+        # staged sweep: after step 0's Gram and visits, launches of half a
+        # step's fused transitions each (beside pair visits), and the last
+        # step's visits + plain update; together they update every executed
+        # pair once (512 m bytes), and the visits' latency hides under them
+        st_n = max(kcnt[4], 1)
+        pairs_per_launch = main_run["visits"] / st_n
+        upd_bytes = pairs_per_launch * 512.0 * n
+        upd_ms = kms[4] / st_n
+        ach = upd_bytes / (upd_ms * 1e-3) / 1e9
+        upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "gpujachx_amd", "data", "upgram_traffic.json")
     if os.path.exists(tfile):
@@ -295,15 +310,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": ("k_flow (dataflow sweep: fused block updates + Grams, pair visits "
-                                "overlapped)") if flow_used else "k_upgram (block update + next-step Gram)",
+                                "overlapped)") if flow_used else
+                               ("k_stage (staged sweep: half a step's fused block updates + next-step "
+                                "Grams beside pair visits)") if staged_used else
+                               "k_upgram (block update + next-step Gram)",
                      "avg_launch_ms": upd_ms,
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},
         "kernels": {"gram": {"avg_ms": gram_ms, "GBs": gram_ach, "tflops": gram_tf},
                     "pair": {"avg_ms": pair_ms},
                     "update": {"avg_ms": upd_ms, "GBs": ach, "tflops": upd_tf},
-                    "launches_per_step": float((kcnt[3] if flow_used else kcnt[2]) / args.steps)},
+                    "launches_per_step": float((kcnt[3] if flow_used else kcnt[4] if staged_used
+                                                else kcnt[2]) / args.steps)},
         "dataflow": flow_used,
+        "staged": staged_used,
         "graph_launched_s": ({"note": "same SVD, one captured hipGraph per sweep, no timing events "
                                       "(the production path); `value` is timed with HIP events "
                                       "around every launch, which needs step-by-step launches",

@@ -96,6 +96,10 @@ def lib():
     L.hj_set_factorization.restype = i
     L.hj_last_dataflow.argtypes = []
     L.hj_last_dataflow.restype = i
+    L.hj_set_staged.argtypes = [i]
+    L.hj_set_staged.restype = i
+    L.hj_last_staged.argtypes = []
+    L.hj_last_staged.restype = i
     L.hj_dbg_scalar.argtypes = [i, i, pd, pd, pd, pd, pd, ctypes.POINTER(ctypes.c_int)]
     L.hj_dbg_scalar.restype = i
     L.hj_version.argtypes = []
@@ -284,6 +288,18 @@ def set_factorization(f):
 def last_dataflow():
     """Whether the last hypJacL2 / hypJacL2_dev call ran dataflow sweeps."""
     return bool(lib().hj_last_dataflow())
+
+
+def set_staged(on):
+    """Staged sweeps on/off (on by default: each step transition as two
+    launches that overlap pair visits with half the block updates); returns
+    the previous setting."""
+    return bool(lib().hj_set_staged(1 if on else 0))
+
+
+def last_staged():
+    """Whether the last hypJacL2 / hypJacL2_dev call ran staged sweeps."""
+    return bool(lib().hj_last_staged())
 
 
 def last_skips():

@@ -17,6 +17,15 @@ constexpr double MYTOL = 0x1.6a09e667f3bcdp-51;      // sqrt(32) * 2^-53
 constexpr double COSHI_FIX = 0x1.279a74590331cp+0;   // 2/sqrt(3), correctly rounded
 constexpr double TANH1_FIX = 0.5;
 
+// x held in a register at this point: the operations producing it are
+// issued here, on every path, rather than moved into a later branch (where
+// they would run after, not beside, the other work of the step)
+__device__ __forceinline__ double pin(double x)
+{
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 __device__ __forceinline__ uint64_t d2u(double x) { return (uint64_t)__double_as_longlong(x); }
 __device__ __forceinline__ double u2d(uint64_t u) { return __longlong_as_double((long long)u); }
 
@@ -94,21 +103,20 @@ __device__ __forceinline__ int dev_rot(bool trig, double Apq, double Dp, double 
                                        double &t)
 {
   const double h = trig ? (Dq - Dp) : (Dq + Dp);
-  const double qz = h / (2.0 * Apq);
+  const double qz = pin(h / (2.0 * Apq));
   const double ct2 = trig ? qz : -qz;
   const double CT2 = fabs(ct2);
   const bool large = CT2 >= SQRT_2_HEPS;
   const bool small = trig && (CT2 < SQRT_HEPS);
   const bool fix = !trig && !large && !(CT2 > 1.0);
-  const double sq = sqrt(__fma_rn(CT2, CT2, trig ? 1.0 : -1.0));
+  const double sq = pin(sqrt(__fma_rn(CT2, CT2, trig ? 1.0 : -1.0)));
   const double cot = large ? 2.0 * CT2 : (small ? CT2 + 1.0 : CT2 + sq);
-  const double tt = fix ? TANH1_FIX : copysign(1.0 / cot, ct2);
-  const double cr = dev_rsqrt(__fma_rn(trig ? tt : -tt, tt, 1.0));
+  const double rcot = pin(1.0 / cot);  // (a statement of its own: a pin inside ?: runs on one arm)
+  const double tt = fix ? TANH1_FIX : copysign(rcot, ct2);
+  const double cr = pin(dev_rsqrt(__fma_rn(trig ? tt : -tt, tt, 1.0)));
   c = large ? 1.0 : (fix ? COSHI_FIX : cr);
   t = tt;
-  if (large) return 0;
-  if (small || fix) return 1;
-  return trig ? (cr < 1.0) : (cr > 1.0);
+  return large ? 0 : ((small || fix) ? 1 : (trig ? (cr < 1.0) : (cr > 1.0)));
 }
 
 __device__ __forceinline__ int dev_rot_t(double Apq, double Dp, double Dq, double &c, double &t)
@@ -198,12 +206,15 @@ __device__ __forceinline__ void chol32_rows(double (&a)[32], unsigned x, double 
       // the LDS accesses of a wavefront complete in program order, and col
       // may alias itself, so the compiler keeps write-then-read order
       col[x] = a[k];
+      // the next pivot A(k+1,k+1) - L(k+1,k)^2 from lane k+1's own row
+      // (L(k+1,k) is its a[k]): the same fma as its column update below,
+      // without waiting for the column's round trip through LDS
+      akk = readlane_d(__fma_rn(-a[k], a[k], a[k + 1u]), k + 1u);
+      const double rn = dev_rsqrt(akk), dn = sqrt(akk);
       {
         const double l1 = col[k + 1u];
         if (x >= k + 1u) a[k + 1u] = __fma_rn(-a[k], l1, a[k + 1u]);
       }
-      akk = readlane_d(a[k + 1u], k + 1u);
-      const double rn = dev_rsqrt(akk), dn = sqrt(akk);
 #pragma unroll
       for (unsigned j = k + 2u; j < 32u; ++j) {
         const double ljk = col[j];

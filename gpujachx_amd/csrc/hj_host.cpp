@@ -27,8 +27,8 @@ struct Stats {
   double sec[HJ_MAX_SWEEP_CAP] = {0.0};
   unsigned skip[HJ_MAX_SWEEP_CAP] = {0u};
   unsigned split[HJ_MAX_SWEEP_CAP] = {0u};
-  double kms[4] = {0.0, 0.0, 0.0, 0.0};
-  unsigned long long kcnt[4] = {0ull, 0ull, 0ull, 0ull};
+  double kms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  unsigned long long kcnt[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
   unsigned long long skipped = 0ull, visits = 0ull, splits = 0ull;
 };
 static Stats g_stats;
@@ -58,6 +58,9 @@ static int g_fact = HJ_FACT_CHOLESKY;
 // dataflow sweeps (one persistent launch per sweep) where the strategy allows
 static int g_flow = 0;
 static int g_last_flow = 0;  // the last run used them
+// staged sweeps (k_stage): on unless switched off
+static int g_stage = 1;
+static int g_last_stage = 0;
 static void *g_trace_buf = nullptr;
 static unsigned g_trace_sweeps = 0u;
 
@@ -175,8 +178,19 @@ struct Work {
   size_t ncnt = 0u;
   unsigned long long *d_trace = nullptr;  // item timestamps (HJ_FLOW_TRACE), else null
   std::vector<FlowItem> hitems;
+  // staged sweep: per transition the next pairs of quads [0, nq/2) then of
+  // the rest; transitions whose first-half quads need a pair of the previous
+  // transition's second half
+  bool stage_ok = false;
+  uint16_t *d_plist = nullptr;
+  std::vector<char> cross;
 
   size_t keys() const { return (size_t)np * steps; }
+  // step parity buffers: the Gram partials a step's visits read, the W and
+  // update flags they write (parity 0 is what the step-by-step path uses)
+  double *part(unsigned par) const { return d_part + (size_t)par * 768u * nchunk * np; }
+  double *W(unsigned par) const { return d_W + (size_t)par * 1024u * np; }
+  unsigned *flags(unsigned par) const { return d_flags + (size_t)par * np; }
 
   void alloc(unsigned nrow, unsigned ncol, bool withV)
   {
@@ -189,9 +203,9 @@ struct Work {
     chV = chG;
     HIP_CALL(hipMalloc(&d_s1, sizeof(uint16_t) * 2u * np * steps));
     HIP_CALL(hipMalloc(&d_s0, STRAT0_TAB_BYTES));
-    HIP_CALL(hipMalloc(&d_part, sizeof(double) * 768u * nchunk * np));
-    HIP_CALL(hipMalloc(&d_W, sizeof(double) * 1024u * np));
-    HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np));
+    HIP_CALL(hipMalloc(&d_part, sizeof(double) * 768u * nchunk * np * 2u));
+    HIP_CALL(hipMalloc(&d_W, sizeof(double) * 1024u * np * 2u));
+    HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np * 2u));
     HIP_CALL(hipMalloc(&d_ctl, sizeof(RunCtl)));
     HIP_CALL(hipHostMalloc(&h_ctl, sizeof(RunCtl), hipHostMallocDefault));
     HIP_CALL(hipMalloc(&d_rec, sizeof(VisitRec) * keys()));
@@ -240,6 +254,41 @@ struct Work {
     HIP_CALL(hipMalloc(&d_quads, sizeof(Quad) * all.size()));
     HIP_CALL(hipMemcpy(d_quads, all.data(), sizeof(Quad) * all.size(), hipMemcpyHostToDevice));
     hquads.swap(all);
+  }
+
+  // The launch plan of a staged sweep (every transition fused, at least two
+  // quads per transition).  Quads [0, h) of a transition form its first
+  // half, h = nq/2.
+  bool build_stage()
+  {
+    stage_ok = false;
+    if (hquads.empty() || steps < 2u || np < 4u) return false;
+    for (unsigned t = 0u; t + 1u < steps; ++t)
+      if (!fuse[t]) return false;
+    const unsigned nq = np / 2u, h = nq / 2u;
+    std::vector<uint16_t> pl((size_t)(steps - 1u) * np);
+    std::vector<char> inB(np);
+    cross.assign(steps, 0);
+    for (unsigned t = 0u; t + 1u < steps; ++t) {
+      const Quad *q = hquads.data() + (size_t)t * nq;
+      for (unsigned i = 0u; i < nq; ++i) {
+        pl[(size_t)t * np + 2u * i] = q[i].n0;
+        pl[(size_t)t * np + 2u * i + 1u] = q[i].n1;
+      }
+      if (t) {
+        const Quad *pq = hquads.data() + (size_t)(t - 1u) * nq;
+        std::fill(inB.begin(), inB.end(), 0);
+        for (unsigned i = h; i < nq; ++i)
+          inB[pq[i].n0] = inB[pq[i].n1] = 1;
+        for (unsigned i = 0u; i < h; ++i)
+          if (inB[q[i].p0] || inB[q[i].p1])
+            cross[t] = 1;
+      }
+    }
+    HIP_CALL(hipMalloc(&d_plist, sizeof(uint16_t) * pl.size()));
+    HIP_CALL(hipMemcpy(d_plist, pl.data(), sizeof(uint16_t) * pl.size(), hipMemcpyHostToDevice));
+    stage_ok = true;
+    return true;
   }
 
   // The item list of a dataflow sweep: the visits of step 0 (two pairs per
@@ -387,6 +436,7 @@ struct Work {
     HIP_CALL(hipFree(d_skip));
     HIP_CALL(hipFree(d_model));
     if (d_quads) HIP_CALL(hipFree(d_quads));
+    if (d_plist) HIP_CALL(hipFree(d_plist));
     if (d_items) HIP_CALL(hipFree(d_items));
     if (d_srcq) HIP_CALL(hipFree(d_srcq));
     if (d_fpart) HIP_CALL(hipFree(d_fpart));
@@ -526,6 +576,99 @@ static void issue_sweep_flow(hipStream_t s, const Work &w, bool sort, unsigned n
                 w.np, w.d_fW + (size_t)ring * w.np * 1024u, w.d_fflags + (size_t)ring * w.np);
 }
 
+// One level-2 sweep as staged launches.  With A_t / B_t the first / second
+// half of the quads of transition t -> t+1 and P_A(t+1) / P_B(t+1) the pairs
+// of step t+1 they form:
+//   Gram of step 0, visits of step 0;
+//   for t = 0 .. S-2:
+//     X_t: fused transition of A_t   + visits of P_B(t) (t >= 1)
+//     Y_t: fused transition of B_t   + visits of P_A(t+1)
+//   visits of P_B(S-1), update of step S-1.
+// A launch's visits need only earlier launches (the Gram partials of their
+// pairs); a fused item needs the W of its two pairs: those of A_t come from
+// P_A(t), visited in Y_{t-1}, unless the quads of A_t take a pair of P_B(t)
+// (cross[t]): then X_t runs as two launches, visits first.  Step parity
+// buffers keep a launch's visits (step t+1) and fused items (step t) apart.
+// Profiling events: one before the first launch and one after each launch.
+static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsigned nswp,
+                                   int definite, unsigned nplus, double *dG, unsigned lddG,
+                                   unsigned nrow, hipEvent_t *ev, bool planned, bool split)
+{
+  const unsigned np = w.np, nq = np / 2u, h = nq / 2u, S = w.steps;
+  const uint8_t *plan = planned ? w.d_skip : nullptr;
+  unsigned nev = 0u;
+  auto mark = [&]() {
+    if (ev) HIP_CALL(hipEventRecord(ev[nev], s));
+    ++nev;
+  };
+  StageArgs base{};
+  base.G = dG;
+  base.ldG = lddG;
+  base.nchunk = w.nchunk;
+  base.CH = w.CH;
+  base.nw = gram_waves(w.CH);
+  base.ctl = w.d_ctl;
+  base.strat0 = w.d_s0;
+  base.s0steps = w.s0steps;
+  base.nswp = nswp;
+  base.definite = definite;
+  base.nplus = nplus;
+  auto visits = [&](StageArgs &x, unsigned st, const uint16_t *list, unsigned n) {
+    x.npl = n;
+    x.npw = (n + 1u) / 2u;
+    x.plist = list;
+    x.ppairs = w.d_s1 + (size_t)st * np * 2u;
+    x.ppart = w.part(st & 1u);
+    x.pW = w.W(st & 1u);
+    x.pflags = w.flags(st & 1u);
+    x.prec = w.d_rec + (size_t)st * np;
+    x.pplan = plan ? plan + (size_t)st * np : nullptr;
+  };
+  auto fused = [&](StageArgs &x, unsigned st, unsigned q0, unsigned n) {
+    x.upairs = w.d_s1 + (size_t)st * np * 2u;
+    x.uW = w.W(st & 1u);
+    x.uflags = w.flags(st & 1u);
+    x.quads = w.d_quads + (size_t)st * nq + q0;
+    x.nquads = n;
+    x.upart = w.part((st + 1u) & 1u);
+    x.uplan = plan ? plan + (size_t)(st + 1u) * np : nullptr;
+  };
+  auto stage = [&](StageArgs &x) { launch_stage(s, sort, x); };
+  mark();
+  launch_gram(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
+  if (split)
+    launch_gram_split(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
+  launch_pair(s, sort, w.d_s1, np, w.part(0u), w.nchunk, w.W(0u), w.flags(0u), w.d_ctl, w.d_rec,
+              plan, w.d_s0, w.s0steps, nswp, definite, nplus);
+  mark();
+  for (unsigned t = 0u; t + 1u < S; ++t) {
+    StageArgs x = base;
+    if (t) {
+      visits(x, t, w.d_plist + (size_t)(t - 1u) * np + 2u * h, np - 2u * h);
+      if (w.cross[t]) {
+        stage(x);
+        mark();
+        x = base;
+      }
+    }
+    fused(x, t, 0u, h);
+    stage(x);
+    mark();
+    StageArgs y = base;
+    visits(y, t + 1u, w.d_plist + (size_t)t * np, 2u * h);
+    fused(y, t, h, nq - h);
+    stage(y);
+    mark();
+  }
+  StageArgs z = base;
+  visits(z, S - 1u, w.d_plist + (size_t)(S - 2u) * np + 2u * h, np - 2u * h);
+  stage(z);
+  launch_update(s, dG, lddG, nrow, w.chG, nullptr, 0u, 0u, w.chV, w.d_s1 + (size_t)(S - 1u) * np * 2u,
+                np, w.W((S - 1u) & 1u), w.flags((S - 1u) & 1u));
+  mark();
+  return nev;
+}
+
 // Sweeps + initD on device data (the body of 1/HypJacL2.cu:104-178).
 //
 // With the learned pair-skip form, every sweep after the first starts with
@@ -560,6 +703,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   const bool flow = g_flow && !full_svd && g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY &&
                     w.build_flow();
   g_last_flow = flow ? 1 : 0;
+  const bool staged = !flow && g_stage && !full_svd && g_step_limit <= 0 &&
+                      g_fact == HJ_FACT_CHOLESKY && w.build_stage();
+  g_last_stage = staged ? 1 : 0;
   if (full_svd)
     launch_initV(s, dV, 0u, ncol, lddV);
   const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
@@ -596,7 +742,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   hipGraphExec_t gexec = nullptr;
   std::vector<hipEvent_t> evs;
   if (g_profile) {
-    evs.resize((size_t)4u * w.steps);
+    evs.resize((size_t)4u * w.steps + 8u);
     for (auto &e : evs) HIP_CALL(hipEventCreate(&e));
   }
   auto sweep_head = [&](hipStream_t st) {
@@ -613,6 +759,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     sweep_head(s);
     if (flow)
       issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned, split);
+    else if (staged)
+      issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned,
+                         split);
     else
       issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
                   nullptr, planned, split, nullptr);
@@ -629,6 +778,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   bool stop = false;
   unsigned issued = 0u;
   std::vector<char> gram_launched(w.steps, 0);
+  unsigned nstage_ev = 0u;
   g_stats.skipped = 0ull;
   while (blk_swp < g_max_sweeps) {
     if (eager) {
@@ -637,6 +787,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       if (flow) {
         issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow,
                          g_profile ? evs.data() : nullptr, planned, split);
+        issued = w.steps;
+      }
+      else if (staged) {
+        nstage_ev = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow,
+                                       g_profile ? evs.data() : nullptr, planned, split);
         issued = w.steps;
       }
       else
@@ -656,6 +811,15 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       HIP_CALL(hipEventElapsedTime(&ms, evs[0], evs[1]));
       g_stats.kms[3] += ms;
       g_stats.kcnt[3] += 1ull;
+    }
+    else if (g_profile && staged) {
+      // kind 4: every launch after step 0's Gram and visits
+      for (unsigned e = 1u; e + 1u < nstage_ev; ++e) {
+        float ms = 0.0f;
+        HIP_CALL(hipEventElapsedTime(&ms, evs[e], evs[e + 1u]));
+        g_stats.kms[4] += ms;
+        g_stats.kcnt[4] += 1ull;
+      }
     }
     else if (g_profile)
       for (unsigned st = 0u; st < issued; ++st)
@@ -956,7 +1120,7 @@ extern "C" int hj_dbg_gram(int split, unsigned nrow, const double *hG, double *h
 
 extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
 {
-  if (kind < 0 || kind > 3) return -1.0;
+  if (kind < 0 || kind > 4) return -1.0;
   if (launches) *launches = g_stats.kcnt[kind];
   return g_stats.kms[kind];
 }
@@ -973,6 +1137,15 @@ extern "C" int hj_set_dataflow(int on)
 }
 
 extern "C" int hj_last_dataflow(void) { return g_last_flow; }
+
+extern "C" int hj_set_staged(int on)
+{
+  const int prev = g_stage;
+  g_stage = on ? 1 : 0;
+  return prev;
+}
+
+extern "C" int hj_last_staged(void) { return g_last_stage; }
 
 extern "C" int hj_set_factorization(int f)
 {

@@ -378,19 +378,19 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
   }
 
   if (live) {
-    // sum of the chunk partials in chunk order: three entries per thread at
-    // a time, all their chunk loads in flight together
+    // sum of the chunk partials in chunk order: the six entries of a thread
+    // with all their chunk loads in flight together (one memory round trip)
     const double *const pp = part + (size_t)pi * nchunk * 768u;
-#pragma unroll 1
-    for (unsigned e0 = tid; e0 < 768u; e0 += 384u) {
-      double v[3][MAX_GRAM_CHUNKS];
+    {
+      const unsigned e0 = tid;
+      double v[6][MAX_GRAM_CHUNKS];
 #pragma unroll
-      for (unsigned q = 0u; q < 3u; ++q)
+      for (unsigned q = 0u; q < 6u; ++q)
 #pragma unroll
         for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
           if (c < nchunk) v[q][c] = pp[(size_t)c * 768u + e0 + 128u * q];
 #pragma unroll
-      for (unsigned q = 0u; q < 3u; ++q) {
+      for (unsigned q = 0u; q < 6u; ++q) {
         const unsigned e = e0 + 128u * q;
         const unsigned tile = e >> 8;
         double s = v[q][0];
@@ -470,6 +470,9 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
   const unsigned y = tid >> 3, r = tid & 7u;
   unsigned blk_s = 0u, blk_b = 0u;
   double my_cos = 0.0;
+  // |Apq| / (|p| |q|) of the previous step, divided during the next step's
+  // loads (off the step's chain of dependent operations)
+  double cos_n = 0.0, cos_d = 0.0;
   bool sweeping = live;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     // the halves go on while either still sweeps
@@ -489,6 +492,10 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
         Vp[i] = RA(W, r + 8u * i, p);
         Vq[i] = RA(W, r + 8u * i, q);
       }
+      if (swp == 0u) {
+        const double cr = pin(cos_n / cos_d);
+        my_cos = fmax(my_cos, (cos_d > 0.0) ? cr : 0.0);
+      }
       double Dp = sum8x4(__fma_rn(Gp[0], Gp[0], 0.0), __fma_rn(Gp[1], Gp[1], 0.0),
                          __fma_rn(Gp[2], Gp[2], 0.0), __fma_rn(Gp[3], Gp[3], 0.0));
       double Dq = sum8x4(__fma_rn(Gq[0], Gq[0], 0.0), __fma_rn(Gq[1], Gq[1], 0.0),
@@ -497,22 +504,25 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
                                 __fma_rn(Gp[2], Gq[2], 0.0), __fma_rn(Gp[3], Gq[3], 0.0));
       const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
       const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
-      if (swp == 0u)
-        my_cos = fmax(my_cos, (Dpq_ > 0.0) ? (Apq_ / Dpq_) : 0.0);
+      cos_n = Apq_;
+      cos_d = Dpq_;
       int pdef = def;
       if (hyp) {
         if ((p < npos) && (q < npos)) pdef = +1;
         else if ((p >= npos) && (q >= npos)) pdef = -1;
         else pdef = 0;
       }
+      bool trig;
+      if (!SORT) trig = !hyp || pdef;
+      else trig = (pdef != 0);
+      // the rotation is computed beside the norms and the test, on every
+      // lane; only a transforming pair uses it
+      double c, t;
+      const int rot_b = dev_rot(trig, Apq, Dp, Dq, c, t);
       int transf_b = 0;
       if (transf_s) {
-        double c, t, t_;
-        bool trig;
-        if (!SORT) trig = !hyp || pdef;
-        else trig = (pdef != 0);
-        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
-        t_ = trig ? -t : t;
+        transf_b = rot_b;
+        const double t_ = trig ? -t : t;
         if (SORT && trig) {
           Dp = __fma_rn(t, Apq, Dp);
           Dq = __fma_rn(t_, Apq, Dq);
@@ -545,6 +555,11 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
       my_b += (unsigned)transf_b;
       }
       __syncthreads();
+    }
+    if (sweeping && swp == 0u) {
+      const double cr = pin(cos_n / cos_d);
+      my_cos = fmax(my_cos, (cos_d > 0.0) ? cr : 0.0);
+      cos_d = 0.0;
     }
     if (sweeping && r == 0u) {
       L.cnt_s[y] = my_s;
@@ -1058,6 +1073,41 @@ void k_flow(FlowArgs A)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Staged sweep: the fused transition of half the quads of step s runs in the
+// same launch as the visits of the pairs of another step whose inputs the
+// previous launch completed, so the visits' latency hides under the fused
+// items' HBM stream.  Workgroups [0, npw) hold two visits each, the rest one
+// (quad, chunk) each.  The launch boundaries order everything else; the
+// arithmetic is that of k_pair / k_upgram.
+template <bool SORT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
+void k_stage(StageArgs A)
+{
+  __shared__ FlowLds S;
+  __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
+  const unsigned tid = threadIdx.x, b = blockIdx.x;
+  if (b < A.npw) {
+    if (tid < STRAT0_TAB_BYTES / 16u)
+      reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(A.strat0)[tid];
+    const unsigned half = tid >> 7, ht = tid & 127u, k = 2u * b + half;
+    const bool present = k < A.npl;
+    const unsigned pi = present ? A.plist[k] : 0u;
+    const uint8_t plan = (present && A.pplan) ? A.pplan[pi] : PLAN_FP64;
+    pair_visit<SORT>(ht, present, pi, S.pair[half], &S.pair[half ^ 1u].go, A.ppairs, A.ppart,
+                     A.nchunk, A.pW, A.pflags, A.ctl, A.prec, plan, S0, A.s0steps, A.nswp,
+                     A.definite, A.nplus);
+  }
+  else {
+    const unsigned i = b - A.npw, qi = i / A.nchunk, c = i - qi * A.nchunk;
+    // a copy in registers: through a reference into global memory its byte
+    // fields would be reloaded after every store of the item
+    const Quad Q = A.quads[qi];
+    upgram_item(S.upg, A.G, A.ldG, A.upairs, A.uW, A.uflags, Q, c, A.nchunk, A.CH, A.upart,
+                A.uplan, A.nw);
+  }
+}
+
 // counters of a flow launch to zero (a kernel, so the sweep graph holds no
 // memset node)
 __global__ void k_flow_reset(unsigned *cnt, size_t n)
@@ -1176,6 +1226,17 @@ void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
     hipLaunchKernelGGL(k_flow<true>, dim3(nblocks), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(k_flow<false>, dim3(nblocks), dim3(256), 0, s, a);
+}
+
+void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
+{
+  const unsigned nb = a.npw + a.nquads * a.nchunk;
+  if (!nb)
+    return;
+  if (sort)
+    hipLaunchKernelGGL(k_stage<true>, dim3(nb), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_stage<false>, dim3(nb), dim3(256), 0, s, a);
 }
 
 unsigned flow_blocks_per_cu()

@@ -129,6 +129,40 @@ struct FlowArgs {
   unsigned long long *trace;  // [nitems][3] clock at ticket, inputs ready, done; or null
 };
 
+// One launch of a staged sweep: the visits of a list of pairs of one step
+// (two per workgroup) beside the fused transition of a range of closed quads
+// of another (nchunk workgroups per quad).  The two parts touch disjoint
+// data: the visits read Gram partials and write W / flags of their step's
+// buffers, the fused items read the W / flags of theirs and write G and the
+// partials of the next step's.
+struct StageArgs {
+  double *G;
+  unsigned ldG;
+  unsigned npl, npw;       // pair visits, their workgroups (ceil(npl / 2))
+  const uint16_t *plist;   // [npl] pair slots of the visits' step
+  const uint16_t *ppairs;  // level-1 pairs of the visits' step
+  const double *ppart;     // Gram partials the visits read
+  double *pW;              // W the visits write
+  unsigned *pflags;        // update flags the visits write
+  VisitRec *prec;          // records of the visits' step
+  const uint8_t *pplan;    // plans of the visits' step, or null
+  const uint16_t *upairs;  // level-1 pairs of the fused items' step
+  const double *uW;        // W of that step
+  const unsigned *uflags;  // update flags of that step
+  const Quad *quads;       // first quad of the range
+  unsigned nquads;
+  unsigned nchunk, CH, nw;
+  double *upart;           // Gram partials of the next step
+  const uint8_t *uplan;    // plans of the next step, or null
+  RunCtl *ctl;
+  const uint8_t *strat0;
+  unsigned s0steps, nswp;
+  int definite;
+  unsigned nplus;
+};
+
+void launch_stage(hipStream_t s, bool sort, const StageArgs &a);
+
 // QR-path visit (hj_qr.hip): factorization of the pair read from G, level-0
 // sweeps (Drmac's for sorted definite blocks when drmac), W and the flags
 void launch_pair_qr(hipStream_t s, bool sort, bool drmac, const double *G, unsigned ldG,

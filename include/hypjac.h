@@ -237,8 +237,9 @@ void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
 
 /* Kernel timing of the last call (HIP events on the work stream, enabled
  * by hj_set_profile(1); graphs are not used while profiling):
- * kind 0 = Gram, 1 = pair, 2 = update, 3 = the dataflow sweep kernel;
- * returns total ms and launch count. */
+ * kind 0 = Gram, 1 = pair, 2 = update, 3 = the dataflow sweep kernel,
+ * 4 = the launches of a staged sweep after step 0's visits (k_stage, and
+ * the last step's update); returns total ms and launch count. */
 void hj_set_profile(int on);
 double hj_kernel_ms(int kind, unsigned long long *launches);
 
@@ -266,6 +267,20 @@ void hj_set_step_limit(long steps);
  * the last hypJacL2 / hypJacL2_dev call used them. */
 int hj_set_dataflow(int on);
 int hj_last_dataflow(void);
+
+/* Staged sweeps (no counterpart in the reference; same role as its
+ * per-step launches, 1/HypJacL2.cu:131-148), on by default: each step
+ * transition runs as two launches, each the fused block update + next Gram
+ * of half the step's closed quads beside the pair visits whose Gram the
+ * previous launch completed, so the visits' latency hides under the block
+ * updates' HBM stream.  The results are bit for bit those of the
+ * step-by-step launches.  Used when on, the dataflow sweep is off, every
+ * step transition forms closed quads, ncol >= 128, V is not accumulated, no
+ * step limit is set and the factorization is Cholesky.  Returns the
+ * previous setting; hj_last_staged() tells whether the last hypJacL2 /
+ * hypJacL2_dev call used them. */
+int hj_set_staged(int on);
+int hj_last_staged(void);
 
 /* Factorization of the block pairs, at run time where the reference chooses
  * at build time (share/device_code_common.hpp:78-85,140-144):

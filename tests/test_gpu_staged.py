@@ -1,0 +1,115 @@
+"""Staged sweeps (each step transition as two launches, half the closed quads'
+fused block updates beside the pair visits the previous launch prepared)
+against the step-by-step launches and the oracle: every element of U, V, D,
+the counters and the sweep count bit for bit, through the C ABI.  The
+strategies here have four transitions per sweep whose first-half quads take
+a pair from the previous transition's second half, so both launch plans of
+a transition run."""
+import numpy as np
+import pytest
+
+import gpujachx_amd as g
+import oracle as o
+from conftest import graded_input, svd_input
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    g.configureGPU(0)
+    yield
+    g.set_staged(True)
+    g.set_form(g.FORM_PLAIN)
+
+
+def run(alg, G, nplus, staged, snp1="cycwor"):
+    g.init_strats_builtin("cycwor", 32, snp1, G.shape[1] // 16)
+    old = g.set_staged(staged)
+    try:
+        r = g.hypJacL2(alg, G, nplus)
+        used = g.last_staged()
+    finally:
+        g.set_staged(old)
+    return r, used
+
+
+def assert_same(a, b, ctx):
+    assert a.ret == 0 and b.ret == 0, ctx
+    assert a.sweeps == b.sweeps, ctx
+    assert (a.rot_s, a.rot_b) == (b.rot_s, b.rot_b), ctx
+    np.testing.assert_array_equal(a.D, b.D, err_msg=ctx)
+    np.testing.assert_array_equal(a.G, b.G, err_msg=ctx)
+
+
+@pytest.mark.parametrize("snp1", ["cycwor", "rowcyc", "colcyc", "cycloc"])
+@pytest.mark.parametrize("alg,kind", [(9, "svd"), (8, "svd"), (1, "svd"), (0, "hyp"), (9, "hyp"),
+                                      (9, "neg")])
+def test_staged_equals_step_by_step(snp1, alg, kind):
+    m, n = 512, 256
+    if kind == "hyp":
+        G, nplus = graded_input(m, n, seed=alg + 3), 100
+    else:
+        s = 10.0 ** (-9.0 * np.random.default_rng(alg).uniform(size=n))
+        G, nplus = svd_input(m, n, s, seed=alg + 7), (n if kind == "svd" else 0)
+    a, used_a = run(alg, G, nplus, True, snp1)
+    b, used_b = run(alg, G, nplus, False, snp1)
+    assert used_a and not used_b
+    assert_same(a, b, "%s alg %d %s" % (snp1, alg, kind))
+
+
+@pytest.mark.parametrize("m,n", [(256, 128), (1024, 512)])
+def test_staged_against_oracle(m, n):
+    G = graded_input(m, n, seed=n)
+    g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+    s0, s1 = g.current_strats()
+    r = g.hypJacL2(9, G, n // 3)
+    assert g.last_staged()
+    ro = o.hypJacL2(9, G, n // 3, s0, s1, nthreads=8)
+    assert r.sweeps == ro["sweeps"] and (r.rot_s, r.rot_b) == (ro["rot_s"], ro["rot_b"])
+    np.testing.assert_array_equal(r.D, ro["D"])
+    np.testing.assert_array_equal(r.G, ro["G"])
+
+
+def test_staged_not_used_where_it_does_not_apply():
+    G = graded_input(256, 128, seed=2)
+    # V accumulated: step-by-step path
+    _, used = run(11, G, 128, True)
+    assert not used
+    # a strategy without closed quads
+    _, used = run(9, G, 128, True, snp1="mmstep")
+    assert not used
+    # fewer than four pairs per step
+    _, used = run(9, graded_input(128, 64, seed=3), 64, True)
+    assert not used
+
+
+@pytest.mark.parametrize("form", ["learned+split", "split"])
+def test_staged_forms_equal_step_by_step(form):
+    # skipped and split visits inside staged launches: the same bits as the
+    # step-by-step launches of the same form
+    import os
+    import gpujachx_amd.learned as learned
+    if "learned" in form and not os.path.exists(learned.DEFAULT_MODEL):
+        pytest.skip("no trained model file")
+    n = 1024
+    s = 10.0 ** (-10.0 * np.random.default_rng(3).uniform(size=n))
+    G = svd_input(n, n, s, seed=31)
+    outs = []
+    for staged in (True, False):
+        if "learned" in form:
+            learned.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
+            g.lib().hj_set_skip_min_visits(0)
+        else:
+            g.set_form(g.FORM_SPLIT_GRAM)
+        try:
+            r, used = run(9, G, n, staged)
+            outs.append((r, used, g.last_skips()[1], g.last_splits()[1]))
+        finally:
+            g.set_form(g.FORM_PLAIN)
+            g.lib().hj_set_skip_min_visits(32000)
+    (a, ua, ska, spa), (b, ub, skb, spb) = outs
+    assert ua and not ub
+    assert (ska, spa) == (skb, spb)
+    assert spa > 0
+    assert_same(a, b, "%s (skipped %d, split %d)" % (form, ska, spa))
