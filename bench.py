@@ -270,13 +270,14 @@ def main():
     tfile = os.path.join(ROOT, "gpujachx_amd", "data", "upgram_traffic.json")
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
-        up = tj["kernels"]["k_upgram"]
+        up = tj["kernels"]["k_stage" if staged_used and "k_stage" in tj["kernels"] else "k_upgram"]
         traffic = up["bytes_per_visit"] * pairs_per_launch * n / tj["n"]
         traffic_src = {"file": "gpujachx_amd/data/upgram_traffic.json",
+                       "kernel": "k_stage" if staged_used and "k_stage" in tj["kernels"] else "k_upgram",
                        "bytes_per_visit_at_n": [up["bytes_per_visit"], tj["n"]],
                        "traffic_over_algorithmic": up["traffic_over_algorithmic"],
-                       "counter_scale": [tj["calibration"]["fetch_scale"],
-                                         tj["calibration"]["write_scale"]]}
+                       "counter_scale": up.get("counter_scale", [tj["calibration"]["fetch_scale"],
+                                                                 tj["calibration"]["write_scale"]])}
 
     out = {
         "metric": METRIC,

@@ -1,7 +1,8 @@
 // hj_pmc -- profiling driver without Python: a device-to-device copy of known
 // size (the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters),
 // then K level-1 steps of an N x N SVD (ALG 9, cycwor/cycwor) on a
-// deterministic pseudo-random matrix in HBM.
+// deterministic pseudo-random matrix in HBM, launched step by step; K = 0:
+// one whole sweep as the library runs it (staged launches).
 //
 //   hj_pmc N K [COPY_MIB]
 //   rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc -- gpujachx_amd/lib/hj_pmc 8192 16
@@ -32,8 +33,8 @@ int main(int argc, char **argv)
   const unsigned n = (unsigned)atoi(argv[1]);
   const long steps = atol(argv[2]);
   const size_t copy_mib = (argc > 3) ? (size_t)atol(argv[3]) : 1024u;
-  if (!n || (n % 64u) || steps < 1) {
-    (void)fprintf(stderr, "N must be a positive multiple of 64, K >= 1\n");
+  if (!n || (n % 64u) || steps < 0) {
+    (void)fprintf(stderr, "N must be a positive multiple of 64, K >= 0\n");
     return 2;
   }
   if (configureGPU(0) < 0)
@@ -68,11 +69,14 @@ int main(int argc, char **argv)
   CHECK(hipMemcpy(dG, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   unsigned sw = 0u;
   unsigned long long gs = 0ull, gb = 0ull;
+  const unsigned cap = steps ? 0u : hj_set_max_sweeps(1u);
   hj_set_step_limit(steps);
   const int rc = hypJacL2_dev(9u, n, n, n, dG, n, nullptr, 0u, dD, &sw, &gs, &gb, nullptr, nullptr);
   hj_set_step_limit(0);
+  if (!steps) hj_set_max_sweeps(cap);
   CHECK(hipDeviceSynchronize());
-  (void)printf("hj_pmc: n=%u steps=%ld rc=%d copy=%zu MiB\n", n, steps, rc, copy_mib);
+  (void)printf("hj_pmc: n=%u steps=%ld (0: one sweep, staged %d) rc=%d copy=%zu MiB\n", n, steps,
+               hj_last_staged(), rc, copy_mib);
   CHECK(hipFree(dG));
   CHECK(hipFree(dD));
   return rc ? 1 : 0;

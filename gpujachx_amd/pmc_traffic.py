@@ -12,7 +12,10 @@ fixed factor on this GPU is corrected by that factor.
     python -m gpujachx_amd.pmc_traffic FETCH.csv WRITE.csv [--copy-mib 1024] [--out FILE]
 
 Writes a JSON file with bytes per launch and per executed pair visit
-(N x 32 columns read and written) for k_upgram, k_update and k_gram.
+(N x 32 columns read and written) for k_upgram, k_update and k_gram; with
+a counter run of one staged sweep (`hj_pmc 8192 0`), also for k_stage,
+whose launches each update half a step's pairs (the few launches with
+visits only are left out of its average).
 """
 import argparse
 import csv
@@ -20,7 +23,7 @@ import json
 import os
 
 KERNELS = {"k_upgram": "hj::k_upgram(", "k_update": "hj::k_update(", "k_gram": "hj::k_gram(",
-           "k_pair": "hj::k_pair"}
+           "k_pair": "hj::k_pair", "k_stage": "hj::k_stage<"}
 
 
 def per_kernel(path):
@@ -44,6 +47,8 @@ def main(argv=None):
     ap.add_argument("write_csv")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--merge", action="store_true",
+                    help="keep the other kernels' entries of an existing output file")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                   "upgram_traffic.json"))
     args = ap.parse_args(argv)
@@ -58,16 +63,29 @@ def main(argv=None):
     for k in KERNELS:
         if k not in f or k not in w:
             continue
-        fk = sum(f[k]) / len(f[k])
-        wk = sum(w[k]) / len(w[k])
+        fl, wl = f[k], w[k]
+        nv = visits
+        if k == "k_stage":
+            # launches with fused items (half a step's pairs each), not the
+            # visits-only launches
+            mf, mw = sorted(fl)[len(fl) // 2], sorted(wl)[len(wl) // 2]
+            fl = [v for v in fl if v > 0.5 * mf]
+            wl = [v for v in wl if v > 0.5 * mw]
+            nv = visits // 2
+        fk = sum(fl) / len(fl)
+        wk = sum(wl) / len(wl)
         b = (fk * fs + wk * ws) * 1024.0
-        alg = {"k_upgram": 512.0, "k_update": 512.0, "k_gram": 256.0}.get(k)
-        ent = {"dispatches": len(f[k]), "fetch_kib": fk, "write_kib": wk,
-               "bytes_per_launch": b, "bytes_per_visit": b / visits}
+        alg = {"k_upgram": 512.0, "k_update": 512.0, "k_gram": 256.0, "k_stage": 512.0}.get(k)
+        ent = {"dispatches": len(fl), "fetch_kib": fk, "write_kib": wk,
+               "bytes_per_launch": b, "bytes_per_visit": b / nv, "counter_scale": [fs, ws]}
         if alg is not None:
-            ent["algorithmic_bytes_per_launch"] = alg * args.n * visits
-            ent["traffic_over_algorithmic"] = b / (alg * args.n * visits)
+            ent["algorithmic_bytes_per_launch"] = alg * args.n * nv
+            ent["traffic_over_algorithmic"] = b / (alg * args.n * nv)
         res["kernels"][k] = ent
+    if args.merge and os.path.exists(args.out):
+        old = json.load(open(args.out))
+        if old.get("n") == args.n:
+            res["kernels"] = {**old["kernels"], **res["kernels"]}
     with open(args.out, "w") as fh:
         json.dump(res, fh, indent=1)
     print(json.dumps(res))
