@@ -1099,12 +1099,18 @@ void k_stage(StageArgs A)
                      A.definite, A.nplus);
   }
   else {
-    const unsigned i = b - A.npw, qi = i / A.nchunk, c = i - qi * A.nchunk;
-    // a copy in registers: through a reference into global memory its byte
-    // fields would be reloaded after every store of the item
-    const Quad Q = A.quads[qi];
-    upgram_item(S.upg, A.G, A.ldG, A.upairs, A.uW, A.uflags, Q, c, A.nchunk, A.CH, A.upart,
-                A.uplan, A.nw);
+    // item workgroup j takes items j, j + niw, j + 2 niw, ...
+    const unsigned nitems = A.nquads * A.nchunk;
+    const unsigned niw = (nitems + A.ipw - 1u) / A.ipw;
+    for (unsigned i = b - A.npw; i < nitems; i += niw) {
+      const unsigned qi = i / A.nchunk, c = i - qi * A.nchunk;
+      // a copy in registers: through a reference into global memory its
+      // byte fields would be reloaded after every store of the item
+      const Quad Q = A.quads[qi];
+      upgram_item(S.upg, A.G, A.ldG, A.upairs, A.uW, A.uflags, Q, c, A.nchunk, A.CH, A.upart,
+                  A.uplan, A.nw);
+      __syncthreads();
+    }
   }
 }
 
@@ -1230,7 +1236,7 @@ void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
 {
-  const unsigned nb = a.npw + a.nquads * a.nchunk;
+  const unsigned nb = a.npw + (a.nquads * a.nchunk + a.ipw - 1u) / a.ipw;
   if (!nb)
     return;
   if (sort)

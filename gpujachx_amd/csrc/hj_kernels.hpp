@@ -152,6 +152,7 @@ struct StageArgs {
   const Quad *quads;       // first quad of the range
   unsigned nquads;
   unsigned nchunk, CH, nw;
+  unsigned ipw;            // fused items per item workgroup (>= 1)
   double *upart;           // Gram partials of the next step
   const uint8_t *uplan;    // plans of the next step, or null
   RunCtl *ctl;
