@@ -290,11 +290,15 @@ def last_dataflow():
     return bool(lib().hj_last_dataflow())
 
 
-def set_staged(on):
-    """Staged sweeps on/off (on by default: each step transition as two
-    launches that overlap pair visits with half the block updates); returns
-    the previous setting."""
-    return bool(lib().hj_set_staged(1 if on else 0))
+STAGED_OFF, STAGED_AUTO, STAGED_ALWAYS = 0, 1, 2
+
+
+def set_staged(mode):
+    """Staged sweeps (each step transition as two launches that overlap pair
+    visits with half the block updates): STAGED_OFF, STAGED_AUTO (default:
+    where a step's block updates move at least 4.8e8 bytes) or
+    STAGED_ALWAYS; True means STAGED_AUTO.  Returns the previous mode."""
+    return lib().hj_set_staged(int(mode))
 
 
 def last_staged():

@@ -58,8 +58,15 @@ static int g_fact = HJ_FACT_CHOLESKY;
 // dataflow sweeps (one persistent launch per sweep) where the strategy allows
 static int g_flow = 0;
 static int g_last_flow = 0;  // the last run used them
-// staged sweeps (k_stage): on unless switched off
+// staged sweeps (k_stage): 0 off, 1 where a step's block updates move at
+// least STAGE_MIN_BYTES, 2 wherever they apply
 static int g_stage = 1;
+// Two staged launches cost about 2 max(Th, Tv) + 22 us per step against
+// 2 Th + Tv step by step (Th: half the step's block updates at the HBM
+// rate, Tv ~ 70 us: a visit's latency): they pay once Th exceeds about
+// 46 us, i.e. 4.8e8 bytes read and written per step (N = 8192: 1.07e9;
+// M = 4096, N = 2048 with V: 2.0e8, measured 86.5 -> 104 ms staged).
+constexpr double STAGE_MIN_BYTES = 4.8e8;
 static int g_last_stage = 0;
 static void *g_trace_buf = nullptr;
 static unsigned g_trace_sweeps = 0u;
@@ -597,8 +604,8 @@ static void issue_sweep_flow(hipStream_t s, const Work &w, bool sort, unsigned n
 // single items: 4.19 s against 4.45 s with two).
 static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsigned nswp,
                                    int definite, unsigned nplus, double *dG, unsigned lddG,
-                                   unsigned nrow, hipEvent_t *ev, bool planned, bool split,
-                                   bool skipping)
+                                   unsigned nrow, double *dV, unsigned lddV, unsigned ncol,
+                                   hipEvent_t *ev, bool planned, bool split, bool skipping)
 {
   const unsigned np = w.np, nq = np / 2u, h = nq / 2u, S = w.steps;
   const uint8_t *plan = planned ? w.d_skip : nullptr;
@@ -630,6 +637,22 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     x.pflags = w.flags(st & 1u);
     x.prec = w.d_rec + (size_t)st * np;
     x.pplan = plan ? plan + (size_t)st * np : nullptr;
+  };
+  // V by the W of step st (all its visits done: those of P_A(st) in Y_{st-1},
+  // of P_B(st) in X_st), in Y_st
+  auto vupdate = [&](StageArgs &x, unsigned st) {
+    if (!dV)
+      return;
+    x.V = dV;
+    x.ldV = lddV;
+    x.nrowV = ncol;
+    x.chV = w.chV;
+    x.nchV = (ncol + w.chV - 1u) / w.chV;
+    x.nvitems = np * x.nchV;
+    x.nvw = (x.nvitems + 3u) / 4u;
+    x.vpairs = w.d_s1 + (size_t)st * np * 2u;
+    x.vW = w.W(st & 1u);
+    x.vflags = w.flags(st & 1u);
   };
   auto fused = [&](StageArgs &x, unsigned st, unsigned q0, unsigned n) {
     x.upairs = w.d_s1 + (size_t)st * np * 2u;
@@ -664,13 +687,14 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     StageArgs y = base;
     visits(y, t + 1u, w.d_plist + (size_t)t * np, 2u * h);
     fused(y, t, h, nq - h);
+    vupdate(y, t);
     stage(y);
     mark();
   }
   StageArgs z = base;
   visits(z, S - 1u, w.d_plist + (size_t)(S - 2u) * np + 2u * h, np - 2u * h);
   stage(z);
-  launch_update(s, dG, lddG, nrow, w.chG, nullptr, 0u, 0u, w.chV, w.d_s1 + (size_t)(S - 1u) * np * 2u,
+  launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, w.d_s1 + (size_t)(S - 1u) * np * 2u,
                 np, w.W((S - 1u) & 1u), w.flags((S - 1u) & 1u));
   mark();
   return nev;
@@ -710,8 +734,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   const bool flow = g_flow && !full_svd && g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY &&
                     w.build_flow();
   g_last_flow = flow ? 1 : 0;
-  const bool staged = !flow && g_stage && !full_svd && g_step_limit <= 0 &&
-                      g_fact == HJ_FACT_CHOLESKY && w.build_stage();
+  const double step_bytes = 512.0 * (ncol / 32u) * ((double)nrow + (full_svd ? ncol : 0u));
+  const bool staged = !flow && g_stage && (g_stage == 2 || step_bytes >= STAGE_MIN_BYTES) &&
+                      g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY && w.build_stage();
   g_last_stage = staged ? 1 : 0;
   if (full_svd)
     launch_initV(s, dV, 0u, ncol, lddV);
@@ -767,8 +792,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     if (flow)
       issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned, split);
     else if (staged)
-      issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned,
-                         split, learned);
+      issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
+                         nullptr, planned, split, learned);
     else
       issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
                   nullptr, planned, split, nullptr);
@@ -797,8 +822,9 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
         issued = w.steps;
       }
       else if (staged) {
-        nstage_ev = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow,
-                                       g_profile ? evs.data() : nullptr, planned, split, learned);
+        nstage_ev = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V,
+                                       lddV, ncol, g_profile ? evs.data() : nullptr, planned,
+                                       split, learned);
         issued = w.steps;
       }
       else
@@ -1145,10 +1171,10 @@ extern "C" int hj_set_dataflow(int on)
 
 extern "C" int hj_last_dataflow(void) { return g_last_flow; }
 
-extern "C" int hj_set_staged(int on)
+extern "C" int hj_set_staged(int mode)
 {
   const int prev = g_stage;
-  g_stage = on ? 1 : 0;
+  g_stage = (mode < 0) ? 0 : (mode > 2) ? 2 : mode;
   return prev;
 }
 

@@ -654,29 +654,12 @@ __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs
 // B operand (G^T, 4 k x 16 rows): lane l gives G(r + l%16, column 4u + l/16);
 // D: lane l, reg q holds output column 16h + 4q + l/16 of row r + l%16.
 // The chain over u = 0..7, k = 0..3 runs over the 32 columns in order.
-__global__ __launch_bounds__(64) void k_update(double *__restrict__ G, unsigned ldG, unsigned nrowG,
-                                               unsigned chG, unsigned nchG,
-                                               double *__restrict__ V, unsigned ldV, unsigned nrowV,
-                                               unsigned chV, unsigned nchV,
-                                               const uint16_t *__restrict__ pairs,
-                                               const double *__restrict__ Wbuf,
-                                               const unsigned *__restrict__ flags)
+// rows [r0, r1) of the two 16-column blocks iBlk, jBlk of M (leading
+// dimension ld) times the pair's 32x32 W, by one wave (lane l)
+__device__ __forceinline__ void update_rows(double *__restrict__ M, unsigned ld, unsigned r0,
+                                            unsigned r1, unsigned iBlk, unsigned jBlk,
+                                            const double *__restrict__ w, unsigned l)
 {
-  const unsigned l = threadIdx.x;
-  const unsigned per = nchG + nchV;
-  const unsigned pi = blockIdx.x / per, c = blockIdx.x - pi * per;
-  if (!flags[pi])
-    return;
-  double *M;
-  unsigned ld, r0, r1;
-  if (c < nchG) {
-    M = G; ld = ldG; r0 = c * chG; r1 = min(r0 + chG, nrowG);
-  }
-  else {
-    M = V; ld = ldV; r0 = (c - nchG) * chV; r1 = min(r0 + chV, nrowV);
-  }
-  const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
-  const double *const w = Wbuf + (size_t)pi * 1024u;
   double a0[8], a1[8];
 #pragma unroll
   for (unsigned u = 0u; u < 8u; ++u) {
@@ -725,6 +708,30 @@ __global__ __launch_bounds__(64) void k_update(double *__restrict__ G, unsigned 
       op1[q][r] = acc1[q];
     }
   }
+}
+
+__global__ __launch_bounds__(64) void k_update(double *__restrict__ G, unsigned ldG, unsigned nrowG,
+                                               unsigned chG, unsigned nchG,
+                                               double *__restrict__ V, unsigned ldV, unsigned nrowV,
+                                               unsigned chV, unsigned nchV,
+                                               const uint16_t *__restrict__ pairs,
+                                               const double *__restrict__ Wbuf,
+                                               const unsigned *__restrict__ flags)
+{
+  const unsigned l = threadIdx.x;
+  const unsigned per = nchG + nchV;
+  const unsigned pi = blockIdx.x / per, c = blockIdx.x - pi * per;
+  if (!flags[pi])
+    return;
+  double *M;
+  unsigned ld, r0, r1;
+  if (c < nchG) {
+    M = G; ld = ldG; r0 = c * chG; r1 = min(r0 + chG, nrowG);
+  }
+  else {
+    M = V; ld = ldV; r0 = (c - nchG) * chV; r1 = min(r0 + chV, nrowV);
+  }
+  update_rows(M, ld, r0, r1, pairs[2u * pi], pairs[2u * pi + 1u], Wbuf + (size_t)pi * 1024u, l);
 }
 
 // ---------------------------------------------------------------------------
@@ -1098,6 +1105,20 @@ void k_stage(StageArgs A)
                      A.nchunk, A.pW, A.pflags, A.ctl, A.prec, plan, S0, A.s0steps, A.nswp,
                      A.definite, A.nplus);
   }
+  else if (b - A.npw >= (A.nquads * A.nchunk + A.ipw - 1u) / A.ipw) {
+    // V items: one (pair, row chunk) per wave
+    const unsigned nitems = A.nquads * A.nchunk;
+    const unsigned j = b - A.npw - (nitems + A.ipw - 1u) / A.ipw;
+    const unsigned v = 4u * j + (tid >> 6);
+    if (v < A.nvitems) {
+      const unsigned pi = v / A.nchV, c = v - pi * A.nchV;
+      if (A.vflags[pi]) {
+        const unsigned r0 = c * A.chV;
+        update_rows(A.V, A.ldV, r0, min(r0 + A.chV, A.nrowV), A.vpairs[2u * pi],
+                    A.vpairs[2u * pi + 1u], A.vW + (size_t)pi * 1024u, tid & 63u);
+      }
+    }
+  }
   else {
     // item workgroup j takes items j, j + niw, j + 2 niw, ...
     const unsigned nitems = A.nquads * A.nchunk;
@@ -1236,7 +1257,7 @@ void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
 {
-  const unsigned nb = a.npw + (a.nquads * a.nchunk + a.ipw - 1u) / a.ipw;
+  const unsigned nb = a.npw + (a.nquads * a.nchunk + a.ipw - 1u) / a.ipw + a.nvw;
   if (!nb)
     return;
   if (sort)

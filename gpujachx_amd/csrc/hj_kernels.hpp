@@ -131,10 +131,12 @@ struct FlowArgs {
 
 // One launch of a staged sweep: the visits of a list of pairs of one step
 // (two per workgroup) beside the fused transition of a range of closed quads
-// of another (nchunk workgroups per quad).  The two parts touch disjoint
-// data: the visits read Gram partials and write W / flags of their step's
-// buffers, the fused items read the W / flags of theirs and write G and the
-// partials of the next step's.
+// of another (nchunk items per quad) and, when V is accumulated, the update
+// of V by the W of a step whose visits are complete.  The parts touch
+// disjoint data: the visits read Gram partials and write W / flags of their
+// step's buffers, the fused items read the W / flags of theirs and write G
+// and the partials of the next step's, the V items read W / flags and write
+// V.
 struct StageArgs {
   double *G;
   unsigned ldG;
@@ -155,6 +157,13 @@ struct StageArgs {
   unsigned ipw;            // fused items per item workgroup (>= 1)
   double *upart;           // Gram partials of the next step
   const uint8_t *uplan;    // plans of the next step, or null
+  // update of V by the W of one step: (pair, row chunk) items, one per wave
+  double *V;
+  unsigned ldV, nrowV, chV, nchV;
+  unsigned nvitems, nvw;   // items (pairs x nchV), their workgroups (ceil(nvitems / 4))
+  const uint16_t *vpairs;  // level-1 pairs of that step
+  const double *vW;        // its W
+  const unsigned *vflags;  // its update flags
   RunCtl *ctl;
   const uint8_t *strat0;
   unsigned s0steps, nswp;

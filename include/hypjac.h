@@ -269,17 +269,20 @@ int hj_set_dataflow(int on);
 int hj_last_dataflow(void);
 
 /* Staged sweeps (no counterpart in the reference; same role as its
- * per-step launches, 1/HypJacL2.cu:131-148), on by default: each step
- * transition runs as two launches, each the fused block update + next Gram
- * of half the step's closed quads beside the pair visits whose Gram the
- * previous launch completed, so the visits' latency hides under the block
- * updates' HBM stream.  The results are bit for bit those of the
- * step-by-step launches.  Used when on, the dataflow sweep is off, every
- * step transition forms closed quads, ncol >= 128, V is not accumulated, no
- * step limit is set and the factorization is Cholesky.  Returns the
- * previous setting; hj_last_staged() tells whether the last hypJacL2 /
- * hypJacL2_dev call used them. */
-int hj_set_staged(int on);
+ * per-step launches, 1/HypJacL2.cu:131-148): each step transition runs as
+ * two launches, each the fused block update + next Gram of half the step's
+ * closed quads beside the pair visits whose Gram the previous launch
+ * completed (and, with V, the update of V), so the visits' latency hides
+ * under the block updates' HBM stream.  The results are bit for bit those
+ * of the step-by-step launches.  mode 0: off; 1 (default): where a step's
+ * block updates read and write at least 4.8e8 bytes (512 (nrow + nV)
+ * ncol/32; below that the visits' latency is longer than half a step's
+ * updates and step-by-step launches are faster); 2: wherever they apply.
+ * They apply when the dataflow sweep is off, every step transition forms
+ * closed quads, ncol >= 128, no step limit is set and the factorization is
+ * Cholesky.  Returns the previous mode; hj_last_staged() tells whether the
+ * last hypJacL2 / hypJacL2_dev call used them. */
+int hj_set_staged(int mode);
 int hj_last_staged(void);
 
 /* Factorization of the block pairs, at run time where the reference chooses

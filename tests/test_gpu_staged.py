@@ -19,13 +19,13 @@ pytestmark = pytest.mark.gpu
 def _device():
     g.configureGPU(0)
     yield
-    g.set_staged(True)
+    g.set_staged(g.STAGED_AUTO)
     g.set_form(g.FORM_PLAIN)
 
 
 def run(alg, G, nplus, staged, snp1="cycwor"):
     g.init_strats_builtin("cycwor", 32, snp1, G.shape[1] // 16)
-    old = g.set_staged(staged)
+    old = g.set_staged(g.STAGED_ALWAYS if staged else g.STAGED_OFF)
     try:
         r = g.hypJacL2(alg, G, nplus)
         used = g.last_staged()
@@ -40,11 +40,14 @@ def assert_same(a, b, ctx):
     assert (a.rot_s, a.rot_b) == (b.rot_s, b.rot_b), ctx
     np.testing.assert_array_equal(a.D, b.D, err_msg=ctx)
     np.testing.assert_array_equal(a.G, b.G, err_msg=ctx)
+    assert (a.V is None) == (b.V is None), ctx
+    if a.V is not None:
+        np.testing.assert_array_equal(a.V, b.V, err_msg=ctx)
 
 
 @pytest.mark.parametrize("snp1", ["cycwor", "rowcyc", "colcyc", "cycloc"])
 @pytest.mark.parametrize("alg,kind", [(9, "svd"), (8, "svd"), (1, "svd"), (0, "hyp"), (9, "hyp"),
-                                      (9, "neg")])
+                                      (9, "neg"), (11, "svd"), (11, "hyp"), (3, "hyp"), (10, "neg")])
 def test_staged_equals_step_by_step(snp1, alg, kind):
     m, n = 512, 256
     if kind == "hyp":
@@ -58,30 +61,41 @@ def test_staged_equals_step_by_step(snp1, alg, kind):
     assert_same(a, b, "%s alg %d %s" % (snp1, alg, kind))
 
 
-@pytest.mark.parametrize("m,n", [(256, 128), (1024, 512)])
-def test_staged_against_oracle(m, n):
+@pytest.mark.parametrize("m,n,alg", [(256, 128, 9), (1024, 512, 9), (256, 128, 11), (512, 256, 3)])
+def test_staged_against_oracle(m, n, alg):
     G = graded_input(m, n, seed=n)
     g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
     s0, s1 = g.current_strats()
-    r = g.hypJacL2(9, G, n // 3)
-    assert g.last_staged()
-    ro = o.hypJacL2(9, G, n // 3, s0, s1, nthreads=8)
+    old = g.set_staged(g.STAGED_ALWAYS)
+    try:
+        r = g.hypJacL2(alg, G, n // 3)
+        assert g.last_staged()
+    finally:
+        g.set_staged(old)
+    ro = o.hypJacL2(alg, G, n // 3, s0, s1, nthreads=8)
     assert r.sweeps == ro["sweeps"] and (r.rot_s, r.rot_b) == (ro["rot_s"], ro["rot_b"])
     np.testing.assert_array_equal(r.D, ro["D"])
     np.testing.assert_array_equal(r.G, ro["G"])
+    if ro["V"] is not None:
+        np.testing.assert_array_equal(r.V, ro["V"])
 
 
 def test_staged_not_used_where_it_does_not_apply():
     G = graded_input(256, 128, seed=2)
-    # V accumulated: step-by-step path
-    _, used = run(11, G, 128, True)
-    assert not used
     # a strategy without closed quads
     _, used = run(9, G, 128, True, snp1="mmstep")
     assert not used
     # fewer than four pairs per step
     _, used = run(9, graded_input(128, 64, seed=3), 64, True)
     assert not used
+    # the default mode: small steps run step by step
+    old = g.set_staged(g.STAGED_AUTO)
+    try:
+        g.init_strats_builtin("cycwor", 32, "cycwor", 128 // 16)
+        g.hypJacL2(9, G, 128)
+        assert not g.last_staged()
+    finally:
+        g.set_staged(old)
 
 
 @pytest.mark.parametrize("form", ["learned+split", "split"])
