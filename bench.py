@@ -109,8 +109,9 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-3,
                     help="split Gram while a pair's last max |cos| exceeds tau")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="skip the unprofiled (graph-launched) timing of the same workload")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the profiled run (HIP events around every launch) that gives the "
+                         "per-kernel breakdown")
     ap.add_argument("--flow", choices=["on", "off"], default="off",
                     help="dataflow sweeps (one persistent launch per sweep) for both forms")
     ap.add_argument("--staged", choices=["on", "off"], default="on",
@@ -172,13 +173,13 @@ def main():
         for _ in range(warmup):
             one()
         g.set_profile(profile)
-        kms, kcnt = np.zeros(5), np.zeros(5)
+        kms, kcnt = np.zeros(6), np.zeros(6)
         visits = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             one()
-            for k in range(5):
+            for k in range(6):
                 ms, c = g.kernel_ms(k)
                 kms[k] += ms
                 kcnt[k] += c
@@ -196,19 +197,18 @@ def main():
                 "split_per_sweep": splits[0], "skip_per_sweep": skips[0], "sigma": sig}
 
     other = "plain" if args.form != "plain" else "learned+split"
-    main_run = timed(args.form, args.steps, args.warmup, True)
-    cmp_run = None if args.no_plain else timed(other, 1, 0, True)
-    # the same workload as the library runs it without profiling (one
-    # captured graph per sweep instead of step-by-step launches with timing
-    # events): a separately timed region, reported beside `value`
-    graph_runs = {}
-    if not args.no_graph:
-        for form in ([args.form] + ([] if args.no_plain else [other])):
-            r = timed(form, 1, 0, False)
-            graph_runs[form] = r["per"]
+    # `value`: the SVDs as the library runs them, one captured graph per
+    # sweep, with HIP events around each sweep's graph launch on the
+    # library's stream (kind 5); a second run of the same SVD with HIP events
+    # around every launch (step-by-step launches) gives the per-kernel
+    # breakdown; the plain form is timed like `value`
+    main_run = timed(args.form, args.steps, args.warmup, False)
+    prof_run = None if args.no_profile else timed(args.form, 1, 0, True)
+    cmp_run = None if args.no_plain else timed(other, 1, 0, False)
     g.set_form(g.FORM_PLAIN)
     per = main_run["per"]
-    kms, kcnt = main_run["kms"], main_run["kcnt"]
+    pk = prof_run if prof_run is not None else main_run
+    kms, kcnt = pk["kms"], pk["kcnt"]
     sweeps = main_run["sweeps"]
     rot_b = main_run["rot_b"]
     sig = main_run["sigma"]
@@ -227,7 +227,7 @@ def main():
     # once per sweep (step 0): 256 m bytes per visit.
     np_ = n // 32
     steps_per_sweep = n // 16 - 1
-    pairs_per_launch = main_run["visits"] / max(kcnt[2], 1)
+    pairs_per_launch = pk["visits"] / max(kcnt[2], 1)
     upd_bytes = pairs_per_launch * 512.0 * n
     gram_bytes = pairs_per_launch * 256.0 * n
     upd_ms = kms[2] / max(kcnt[2], 1)
@@ -249,21 +249,27 @@ def main():
         # a sweep (the last step's update is a k_update after it) and every
         # pair visit; its roofline is the HBM stream of those transitions
         fl_n = max(kcnt[3], 1)
-        pairs_per_launch = (main_run["visits"] / fl_n) * (steps_per_sweep - 1) / steps_per_sweep
+        pairs_per_launch = (pk["visits"] / fl_n) * (steps_per_sweep - 1) / steps_per_sweep
         upd_bytes = pairs_per_launch * 512.0 * n
         upd_ms = kms[3] / fl_n
         ach = upd_bytes / (upd_ms * 1e-3) / 1e9
         upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     staged_used = bool(main_run["staged"])
+    timing_src = "profiled run: HIP events around every launch"
     if staged_used:
         # staged sweep: after step 0's Gram and visits, launches of half a
         # step's fused transitions each (beside pair visits), and the last
         # step's visits + plain update; together they update every executed
-        # pair once (512 m bytes), and the visits' latency hides under them
-        st_n = max(kcnt[4], 1)
+        # pair once (512 m bytes), and the visits' latency hides under them.
+        # Timed in the `value` run itself: the sweeps' GPU time (events
+        # around each graph launch) over their staged launches
+        st_n = max(main_run["kcnt"][5], 1)
         pairs_per_launch = main_run["visits"] / st_n
         upd_bytes = pairs_per_launch * 512.0 * n
-        upd_ms = kms[4] / st_n
+        upd_ms = main_run["kms"][5] / st_n
+        timing_src = ("the timed region: HIP events around each sweep's graph launch, the sweeps' "
+                      "GPU time over their k_stage launches (step 0's Gram and visits, the plan "
+                      "and the last update, about 1 % of a sweep, included)")
         ach = upd_bytes / (upd_ms * 1e-3) / 1e9
         upd_tf = pairs_per_launch * (2048.0 + 1536.0) * n / (upd_ms * 1e-3) / 1e12
     traffic, traffic_src = None, None
@@ -315,20 +321,23 @@ def main():
                                ("k_stage (staged sweep: half a step's fused block updates + next-step "
                                 "Grams beside pair visits)") if staged_used else
                                "k_upgram (block update + next-step Gram)",
-                     "avg_launch_ms": upd_ms,
+                     "avg_launch_ms": upd_ms, "timing": timing_src,
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "mfma_tflops": upd_tf, "mfma_frac_fp64": upd_tf / FP64_PEAK_TFS},
         "kernels": {"gram": {"avg_ms": gram_ms, "GBs": gram_ach, "tflops": gram_tf},
                     "pair": {"avg_ms": pair_ms},
                     "update": {"avg_ms": upd_ms, "GBs": ach, "tflops": upd_tf},
-                    "launches_per_step": float((kcnt[3] if flow_used else kcnt[4] if staged_used
-                                                else kcnt[2]) / args.steps)},
+                    "stage": {"avg_ms": kms[4] / max(kcnt[4], 1)},
+                    "launches_per_svd": float(main_run["kcnt"][5] / max(args.steps, 1) if staged_used
+                                              else kcnt[3] if flow_used else kcnt[2]),
+                    "source": "profiled run (HIP events around every launch)" if prof_run is not None
+                              else "the timed run"},
         "dataflow": flow_used,
         "staged": staged_used,
-        "graph_launched_s": ({"note": "same SVD, one captured hipGraph per sweep, no timing events "
-                                      "(the production path); `value` is timed with HIP events "
-                                      "around every launch, which needs step-by-step launches",
-                              **graph_runs} if graph_runs else None),
+        "profiled_s": ({"note": "the same SVD launched step by step with HIP events around "
+                                "every launch (the per-kernel breakdown's run); `value` is the "
+                                "library's graph-launched path",
+                        args.form: prof_run["per"]} if prof_run is not None else None),
     }
     if cmp_run is not None:
         cs = np.sort(cmp_run["sigma"])[::-1]

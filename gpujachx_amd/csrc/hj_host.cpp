@@ -27,8 +27,9 @@ struct Stats {
   double sec[HJ_MAX_SWEEP_CAP] = {0.0};
   unsigned skip[HJ_MAX_SWEEP_CAP] = {0u};
   unsigned split[HJ_MAX_SWEEP_CAP] = {0u};
-  double kms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  unsigned long long kcnt[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
+  // kinds 0-4: per-launch events (profiling); 5: graph-launched sweeps
+  double kms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  unsigned long long kcnt[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
   unsigned long long skipped = 0ull, visits = 0ull, splits = 0ull;
 };
 static Stats g_stats;
@@ -772,6 +773,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   }
   const bool eager = g_profile || (g_step_limit > 0) || getenv("HJ_NOGRAPH");
   hipGraphExec_t gexec = nullptr;
+  // graph path: events around each sweep's graph launch, and the launches
+  // of the sweep's dominant kind it holds (staged launches; else the steps'
+  // update-class launches; one dataflow launch)
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  unsigned long long graph_launches = 0ull;
   std::vector<hipEvent_t> evs;
   if (g_profile) {
     evs.resize((size_t)4u * w.steps + 8u);
@@ -789,18 +795,22 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     hipGraph_t graph;
     HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     sweep_head(s);
-    if (flow)
+    if (flow) {
       issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned, split);
+      graph_launches = 1ull;
+    }
     else if (staged)
-      issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol,
-                         nullptr, planned, split, learned);
+      graph_launches = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V,
+                                          lddV, ncol, nullptr, planned, split, learned) - 2u;
     else
-      issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
-                  nullptr, planned, split, nullptr);
+      graph_launches = issue_sweep(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV,
+                                   ncol, nullptr, nullptr, planned, split, nullptr);
     HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamEndCapture(s, &graph));
     HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     HIP_CALL(hipGraphDestroy(graph));
+    HIP_CALL(hipEventCreate(&gev[0]));
+    HIP_CALL(hipEventCreate(&gev[1]));
   }
 
   long long swp_tim = 0;
@@ -836,9 +846,18 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
         stop = true;
       HIP_CALL(hipMemcpyAsync(w.h_ctl, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     }
-    else
+    else {
+      HIP_CALL(hipEventRecord(gev[0], s));
       HIP_CALL(hipGraphLaunch(gexec, s));
+      HIP_CALL(hipEventRecord(gev[1], s));
+    }
     HIP_CALL(hipStreamSynchronize(s));
+    if (!eager) {
+      float ms = 0.0f;
+      HIP_CALL(hipEventElapsedTime(&ms, gev[0], gev[1]));
+      g_stats.kms[5] += ms;
+      g_stats.kcnt[5] += graph_launches;
+    }
     if (g_profile && flow) {
       float ms = 0.0f;
       HIP_CALL(hipEventElapsedTime(&ms, evs[0], evs[1]));
@@ -903,6 +922,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       break;
   }
   if (gexec) HIP_CALL(hipGraphExecDestroy(gexec));
+  for (hipEvent_t e : gev)
+    if (e) HIP_CALL(hipEventDestroy(e));
   for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
 
   *glbSwp = blk_swp;
@@ -1153,7 +1174,7 @@ extern "C" int hj_dbg_gram(int split, unsigned nrow, const double *hG, double *h
 
 extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
 {
-  if (kind < 0 || kind > 4) return -1.0;
+  if (kind < 0 || kind > 5) return -1.0;
   if (launches) *launches = g_stats.kcnt[kind];
   return g_stats.kms[kind];
 }

@@ -235,11 +235,15 @@ int hj_mlp_train(const float *X, const float *y, unsigned n, unsigned epochs,
 void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
                     double *sweep_seconds);
 
-/* Kernel timing of the last call (HIP events on the work stream, enabled
- * by hj_set_profile(1); graphs are not used while profiling):
+/* Kernel timing of the last call (HIP events on the work stream).  With
+ * hj_set_profile(1) (graphs are not used while profiling), per launch:
  * kind 0 = Gram, 1 = pair, 2 = update, 3 = the dataflow sweep kernel,
  * 4 = the launches of a staged sweep after step 0's visits (k_stage, and
- * the last step's update); returns total ms and launch count. */
+ * the last step's update).  Without profiling, kind 5 = the graph-launched
+ * sweeps: their GPU time (events around each graph launch) and the
+ * launches of each sweep's dominant kind they hold (staged launches, else
+ * one update-class launch per step, else one dataflow launch).  Returns
+ * total ms and launch count. */
 void hj_set_profile(int on);
 double hj_kernel_ms(int kind, unsigned long long *launches);
 
