@@ -127,3 +127,49 @@ def test_staged_forms_equal_step_by_step(form):
     assert (ska, spa) == (skb, spb)
     assert spa > 0
     assert_same(a, b, "%s (skipped %d, split %d)" % (form, ska, spa))
+
+
+@pytest.mark.parametrize("form,sweeps", [("plain", 1), ("learned+split", 3)])
+def test_staged_full_size(form, sweeps):
+    # N = 8192, the bench's size and launch plan (256 pairs per step, the
+    # cross transitions where the strategy puts them): the first sweeps of
+    # the staged and the step-by-step launches, every bit of G and D equal
+    import ctypes
+    import os
+    import torch
+    import gpujachx_amd.learned as learned
+    if "learned" in form and not os.path.exists(learned.DEFAULT_MODEL):
+        pytest.skip("no trained model file")
+    n = 8192
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    scale = torch.pow(10.0, -6.0 * torch.rand(n, generator=gen, device="cuda", dtype=torch.float64))
+    G0 = torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64) * scale[:, None]
+    g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+    outs = []
+    old_cap = g.set_max_sweeps(sweeps)
+    try:
+        for mode in (g.STAGED_ALWAYS, g.STAGED_OFF):
+            if "learned" in form:
+                learned.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
+            old = g.set_staged(mode)
+            try:
+                W = G0.clone()
+                D = torch.empty(n, dtype=torch.float64, device="cuda")
+                sw = ctypes.c_uint(0)
+                gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+                rc = g.lib().hypJacL2_dev(9, n, n, n, W.data_ptr(), n, None, 0, D.data_ptr(),
+                                          ctypes.byref(sw), ctypes.byref(gs), ctypes.byref(gb), None,
+                                          torch.cuda.current_stream().cuda_stream)
+                torch.cuda.synchronize()
+                assert rc == 0
+                outs.append((W, D, sw.value, gs.value, gb.value, g.last_staged(), g.last_skips()[1]))
+            finally:
+                g.set_staged(old)
+                g.set_form(g.FORM_PLAIN)
+    finally:
+        g.set_max_sweeps(old_cap)
+    (Wa, Da, *ra), (Wb, Db, *rb) = outs
+    assert ra[3] and not rb[3]
+    assert ra[:3] == rb[:3] and ra[4] == rb[4], (ra, rb)
+    assert torch.equal(Da, Db)
+    assert torch.equal(Wa, Wb)
