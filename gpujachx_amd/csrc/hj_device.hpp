@@ -70,23 +70,22 @@ __device__ __forceinline__ double dev_rsqrt(double x)
 {
   // special inputs resolved by selects (no branches: the callers' code stays
   // straight-line for the scheduler); the main path also runs on them, its
-  // value discarded
-  const uint64_t b0 = d2u(x);
-  const bool special = ((b0 >> 52) - 1u) >= 0x7feu;  // not a positive normal number
-  const bool subn = special && (x > 0.0) && (b0 != 0x7ff0000000000000ull);
-  const double xs = subn ? x * 0x1p108 : x;  // positive subnormal: scaled into range
-  const int kadj = subn ? 54 : 0;
-  const uint64_t b = d2u(xs);
-  const int e = (int)((b >> 52) & 0x7ffu) - 1023;
-  const int k = (e >= 0) ? (e / 2) : -((1 - e) / 2);
-  const double m = u2d((b & 0x000fffffffffffffull) | ((uint64_t)(1023 + e - 2 * k) << 52));
+  // value discarded.  x = mt * 2^e with mt in [0.5, 1) (v_frexp_*, subnormal
+  // x included), then x = m * 4^k with m = mt * 2^(2 - e mod 2) in [1, 4)
+  const bool special = !((x > 0.0) && (x < __builtin_inf()));  // not positive finite
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  const double mt = __builtin_amdgcn_frexp_mant(x);
+  const int odd = e & 1;
+  const double m = __builtin_amdgcn_ldexp(mt, 2 - odd);
+  const int k = (e - 2 + odd) >> 1;  // exact: e - 2 + odd is even
   const double y = rsqrt_core(m);
-  const double normal = y * u2d((uint64_t)(1023 + kadj - k) << 52);
+  const double normal = __builtin_amdgcn_ldexp(y, -k);  // exact: the result is a normal number
+  const uint64_t b0 = d2u(x);
   double sp = 0.0;                                                   // +inf
   sp = (x < 0.0) ? u2d(0xfff8000000000000ull) : sp;                  // negative: NaN
   sp = (x == 0.0) ? u2d((b0 & 0x8000000000000000ull) | 0x7ff0000000000000ull) : sp;  // +-0: +-inf
   sp = (x != x) ? x + x : sp;                                        // NaN
-  return (special && !subn) ? sp : normal;
+  return special ? sp : normal;
 }
 
 // 2x2 rotation of share/device_code_common_rotate.hpp:189-237 (USE_RSQRT),

@@ -53,7 +53,11 @@ def test_scalar_kernels_bitwise():
     m = rng.uniform(1.0, 4.0, 1 << 20)
     edge = np.array([np.nextafter(4.0 ** k, 0.0) for k in range(-3, 4)] +
                     [np.nextafter(4.0 ** k, np.inf) for k in range(-3, 4)])
-    x = np.concatenate([x, m, m * 2.0 ** -700, edge])
+    # subnormals of every exponent, the extremes, special values
+    sub = rng.uniform(1.0, 2.0, 52 * 8) * np.repeat(2.0 ** -np.arange(1023, 1075, dtype=float), 8)
+    spec = np.array([0.0, -0.0, -1.0, -5e-324, np.inf, -np.inf, np.nan, 5e-324, 1e-310,
+                     np.nextafter(2.0 ** -1022, 0.0), 2.0 ** -1022, np.finfo(float).max])
+    x = np.concatenate([x, m, m * 2.0 ** -700, edge, sub, spec])
     c, _, _ = g.dbg_scalar(0, x)
     np.testing.assert_array_equal(c, o.rsqrt_n(x))
     # rotations across all branches: |ct2| tiny, moderate, huge, around 1
