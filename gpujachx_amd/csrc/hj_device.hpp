@@ -157,8 +157,10 @@ __device__ __forceinline__ double sum32(double v)
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v)
 {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  // every lane reads a lane of its own row (quad / half-row permutations):
+  // no lane keeps an old value, so none is set up
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double lane_xor4(double v) { return dpp_mov<0x1B>(dpp_mov<0x141>(v)); }
