@@ -1084,9 +1084,11 @@ void k_flow(FlowArgs A)
 // Staged sweep: the fused transition of half the quads of step s runs in the
 // same launch as the visits of the pairs of another step whose inputs the
 // previous launch completed, so the visits' latency hides under the fused
-// items' HBM stream.  Workgroups [0, npw) hold two visits each, the rest one
-// (quad, chunk) each.  The launch boundaries order everything else; the
-// arithmetic is that of k_pair / k_upgram.
+// items' HBM stream.  Workgroups [0, npw) hold two visits each; then come
+// the item workgroups, each taking ipw (quad, chunk) items, and, with V,
+// the V workgroups (one (pair, row chunk) update of V per wave).  The
+// launch boundaries order everything else; the arithmetic is that of
+// k_pair / k_upgram / k_update.
 template <bool SORT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
 void k_stage(StageArgs A)
