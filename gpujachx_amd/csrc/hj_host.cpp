@@ -600,9 +600,11 @@ static void issue_sweep_flow(hipStream_t s, const Work &w, bool sort, unsigned n
 // Profiling events: one before the first launch and one after each launch.
 // Fused items per item workgroup: two where every item runs (the item
 // workgroups and the visits' then all start at once, one round of at most
-// two per CU: plain N = 8192 4.99 -> 4.88 s), one with the learned skip
-// (skipped items make a static pairing uneven; the dispatcher balances
-// single items: 4.19 s against 4.45 s with two).
+// two per CU: plain N = 8192 4.99 -> 4.88 s; filling every free slot
+// instead, some workgroups taking two items, is slower: 140 against 124 us
+// per launch), one with the learned skip (skipped items make a static
+// pairing uneven; the dispatcher balances single items: 4.19 s against
+// 4.45 s with two).
 static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsigned nswp,
                                    int definite, unsigned nplus, double *dG, unsigned lddG,
                                    unsigned nrow, double *dV, unsigned lddV, unsigned ncol,
@@ -621,7 +623,6 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   base.nchunk = w.nchunk;
   base.CH = w.CH;
   base.nw = gram_waves(w.CH);
-  base.ipw = skipping ? 1u : 2u;
   base.ctl = w.d_ctl;
   base.strat0 = w.d_s0;
   base.s0steps = w.s0steps;
@@ -661,6 +662,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     x.uflags = w.flags(st & 1u);
     x.quads = w.d_quads + (size_t)st * nq + q0;
     x.nquads = n;
+    x.niw = skipping ? n * w.nchunk : (n * w.nchunk + 1u) / 2u;
     x.upart = w.part((st + 1u) & 1u);
     x.uplan = plan ? plan + (size_t)(st + 1u) * np : nullptr;
   };

@@ -1085,7 +1085,7 @@ void k_flow(FlowArgs A)
 // same launch as the visits of the pairs of another step whose inputs the
 // previous launch completed, so the visits' latency hides under the fused
 // items' HBM stream.  Workgroups [0, npw) hold two visits each; then come
-// the item workgroups, each taking ipw (quad, chunk) items, and, with V,
+// the niw item workgroups, j taking items j, j + niw, ..., and, with V,
 // the V workgroups (one (pair, row chunk) update of V per wave).  The
 // launch boundaries order everything else; the arithmetic is that of
 // k_pair / k_upgram / k_update.
@@ -1107,10 +1107,9 @@ void k_stage(StageArgs A)
                      A.nchunk, A.pW, A.pflags, A.ctl, A.prec, plan, S0, A.s0steps, A.nswp,
                      A.definite, A.nplus);
   }
-  else if (b - A.npw >= (A.nquads * A.nchunk + A.ipw - 1u) / A.ipw) {
+  else if (b - A.npw >= A.niw) {
     // V items: one (pair, row chunk) per wave
-    const unsigned nitems = A.nquads * A.nchunk;
-    const unsigned j = b - A.npw - (nitems + A.ipw - 1u) / A.ipw;
+    const unsigned j = b - A.npw - A.niw;
     const unsigned v = 4u * j + (tid >> 6);
     if (v < A.nvitems) {
       const unsigned pi = v / A.nchV, c = v - pi * A.nchV;
@@ -1124,7 +1123,7 @@ void k_stage(StageArgs A)
   else {
     // item workgroup j takes items j, j + niw, j + 2 niw, ...
     const unsigned nitems = A.nquads * A.nchunk;
-    const unsigned niw = (nitems + A.ipw - 1u) / A.ipw;
+    const unsigned niw = A.niw;
     for (unsigned i = b - A.npw; i < nitems; i += niw) {
       const unsigned qi = i / A.nchunk, c = i - qi * A.nchunk;
       // a copy in registers: through a reference into global memory its
@@ -1259,7 +1258,7 @@ void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
 {
-  const unsigned nb = a.npw + (a.nquads * a.nchunk + a.ipw - 1u) / a.ipw + a.nvw;
+  const unsigned nb = a.npw + a.niw + a.nvw;
   if (!nb)
     return;
   if (sort)

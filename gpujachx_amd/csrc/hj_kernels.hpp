@@ -154,7 +154,7 @@ struct StageArgs {
   const Quad *quads;       // first quad of the range
   unsigned nquads;
   unsigned nchunk, CH, nw;
-  unsigned ipw;            // fused items per item workgroup (>= 1)
+  unsigned niw;            // item workgroups (1..nquads*nchunk; 0 without items)
   double *upart;           // Gram partials of the next step
   const uint8_t *uplan;    // plans of the next step, or null
   // update of V by the W of one step: (pair, row chunk) items, one per wave
