@@ -191,6 +191,7 @@ struct Work {
   // transition's second half
   bool stage_ok = false;
   uint16_t *d_plist = nullptr;
+  uint16_t *d_plist_run = nullptr;  // d_plist reordered per sweep (learned skip)
   std::vector<char> cross;
 
   size_t keys() const { return (size_t)np * steps; }
@@ -295,6 +296,7 @@ struct Work {
     }
     HIP_CALL(hipMalloc(&d_plist, sizeof(uint16_t) * pl.size()));
     HIP_CALL(hipMemcpy(d_plist, pl.data(), sizeof(uint16_t) * pl.size(), hipMemcpyHostToDevice));
+    HIP_CALL(hipMalloc(&d_plist_run, sizeof(uint16_t) * pl.size()));
     stage_ok = true;
     return true;
   }
@@ -445,6 +447,7 @@ struct Work {
     HIP_CALL(hipFree(d_model));
     if (d_quads) HIP_CALL(hipFree(d_quads));
     if (d_plist) HIP_CALL(hipFree(d_plist));
+    if (d_plist_run) HIP_CALL(hipFree(d_plist_run));
     if (d_items) HIP_CALL(hipFree(d_items));
     if (d_srcq) HIP_CALL(hipFree(d_srcq));
     if (d_fpart) HIP_CALL(hipFree(d_fpart));
@@ -667,7 +670,11 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     x.uplan = plan ? plan + (size_t)(st + 1u) * np : nullptr;
   };
   auto stage = [&](StageArgs &x) { launch_stage(s, sort, x); };
+  // with the learned skip, each half-list's running visits first
+  const uint16_t *const plist = skipping ? w.d_plist_run : w.d_plist;
   mark();
+  if (skipping)
+    launch_order_visits(s, w.d_plist, w.d_plist_run, w.d_skip, np, S, 2u * h);
   launch_gram(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
   if (split)
     launch_gram_split(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
@@ -677,7 +684,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   for (unsigned t = 0u; t + 1u < S; ++t) {
     StageArgs x = base;
     if (t) {
-      visits(x, t, w.d_plist + (size_t)(t - 1u) * np + 2u * h, np - 2u * h);
+      visits(x, t, plist + (size_t)(t - 1u) * np + 2u * h, np - 2u * h);
       if (w.cross[t]) {
         stage(x);
         mark();
@@ -688,14 +695,14 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     stage(x);
     mark();
     StageArgs y = base;
-    visits(y, t + 1u, w.d_plist + (size_t)t * np, 2u * h);
+    visits(y, t + 1u, plist + (size_t)t * np, 2u * h);
     fused(y, t, h, nq - h);
     vupdate(y, t);
     stage(y);
     mark();
   }
   StageArgs z = base;
-  visits(z, S - 1u, w.d_plist + (size_t)(S - 2u) * np + 2u * h, np - 2u * h);
+  visits(z, S - 1u, plist + (size_t)(S - 2u) * np + 2u * h, np - 2u * h);
   stage(z);
   launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, w.d_s1 + (size_t)(S - 1u) * np * 2u,
                 np, w.W((S - 1u) & 1u), w.flags((S - 1u) & 1u));

@@ -1136,6 +1136,62 @@ void k_stage(StageArgs A)
   }
 }
 
+// The visit lists of a staged sweep in the order the learned skip makes
+// cheapest: per transition t and half (A: entries [0, h2), B: the rest) of
+// the pair list, the pairs of step t+1 that run before the skipped ones
+// (stable), so that the visit workgroups holding only skipped visits end at
+// once and free their slots for fused items.  One workgroup of 256 per
+// (transition, half), over the half in chunks of 256 entries.
+__global__ __launch_bounds__(256) void k_order_visits(const uint16_t *__restrict__ plist,
+                                                      uint16_t *__restrict__ out,
+                                                      const uint8_t *__restrict__ plan,
+                                                      unsigned np, unsigned h2)
+{
+  __shared__ unsigned run[256];
+  __shared__ unsigned total;
+  const unsigned t = blockIdx.x >> 1, half = blockIdx.x & 1u, tid = threadIdx.x;
+  const unsigned off = t * np + (half ? h2 : 0u), n = half ? np - h2 : h2;
+  const uint8_t *const pl = plan + (size_t)(t + 1u) * np;
+  if (tid == 0u)
+    total = 0u;
+  __syncthreads();
+  // running visits of the half
+  for (unsigned c0 = 0u; c0 < n; c0 += 256u) {
+    const unsigned i = c0 + tid;
+    if (i < n && pl[plist[off + i]] != PLAN_SKIP)
+      atomicAdd(&total, 1u);
+  }
+  __syncthreads();
+  const unsigned nrun = total;
+  unsigned base = 0u;  // running visits in the chunks before this one
+  for (unsigned c0 = 0u; c0 < n; c0 += 256u) {
+    const unsigned i = c0 + tid;
+    const unsigned pi = (i < n) ? plist[off + i] : 0u;
+    const unsigned f = (i < n && pl[pi] != PLAN_SKIP) ? 1u : 0u;
+    // inclusive prefix sum of f over the chunk (Hillis-Steele in LDS)
+    run[tid] = f;
+    __syncthreads();
+    for (unsigned d = 1u; d < 256u; d <<= 1) {
+      const unsigned v = (tid >= d) ? run[tid - d] : 0u;
+      __syncthreads();
+      run[tid] += v;
+      __syncthreads();
+    }
+    const unsigned before = base + run[tid] - f;  // running entries before i
+    if (i < n)
+      out[off + (f ? before : nrun + (i - before))] = (uint16_t)pi;
+    base += run[255];
+    __syncthreads();  // run[] is rewritten by the next chunk
+  }
+}
+
+void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,
+                         unsigned np, unsigned steps, unsigned h2)
+{
+  hipLaunchKernelGGL(k_order_visits, dim3(2u * (steps - 1u)), dim3(256), 0, s, plist, out, plan, np,
+                     h2);
+}
+
 // counters of a flow launch to zero (a kernel, so the sweep graph holds no
 // memset node)
 __global__ void k_flow_reset(unsigned *cnt, size_t n)

@@ -172,6 +172,9 @@ struct StageArgs {
 };
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a);
+// visit lists of a staged sweep, running pairs first (learned skip)
+void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,
+                         unsigned np, unsigned steps, unsigned h2);
 
 // QR-path visit (hj_qr.hip): factorization of the pair read from G, level-0
 // sweeps (Drmac's for sorted definite blocks when drmac), W and the flags
