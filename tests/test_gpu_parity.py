@@ -83,7 +83,7 @@ def test_one_step_every_alg(alg, kind):
 
 @pytest.mark.parametrize("alg", [9, 11])
 def test_c1_svd_full(alg):
-    # M=N=256, s_i = 10^(-6(i-1)/255), cycwor/cycwor
+    # the plain-method SVD workload: M=N=256, s_i = 10^(-6(i-1)/255), cycwor/cycwor
     n = 256
     s = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
     G = svd_input(n, n, s, seed=1)
