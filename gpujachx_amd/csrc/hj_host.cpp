@@ -192,6 +192,7 @@ struct Work {
   bool stage_ok = false;
   uint16_t *d_plist = nullptr;
   uint16_t *d_plist_run = nullptr;  // d_plist reordered per sweep (learned skip)
+  uint16_t *d_qord = nullptr;       // per transition, the quads in item order (learned skip)
   std::vector<char> cross;
 
   size_t keys() const { return (size_t)np * steps; }
@@ -297,6 +298,7 @@ struct Work {
     HIP_CALL(hipMalloc(&d_plist, sizeof(uint16_t) * pl.size()));
     HIP_CALL(hipMemcpy(d_plist, pl.data(), sizeof(uint16_t) * pl.size(), hipMemcpyHostToDevice));
     HIP_CALL(hipMalloc(&d_plist_run, sizeof(uint16_t) * pl.size()));
+    HIP_CALL(hipMalloc(&d_qord, sizeof(uint16_t) * (size_t)(steps - 1u) * (np / 2u)));
     stage_ok = true;
     return true;
   }
@@ -448,6 +450,7 @@ struct Work {
     if (d_quads) HIP_CALL(hipFree(d_quads));
     if (d_plist) HIP_CALL(hipFree(d_plist));
     if (d_plist_run) HIP_CALL(hipFree(d_plist_run));
+    if (d_qord) HIP_CALL(hipFree(d_qord));
     if (d_items) HIP_CALL(hipFree(d_items));
     if (d_srcq) HIP_CALL(hipFree(d_srcq));
     if (d_fpart) HIP_CALL(hipFree(d_fpart));
@@ -663,7 +666,8 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     x.upairs = w.d_s1 + (size_t)st * np * 2u;
     x.uW = w.W(st & 1u);
     x.uflags = w.flags(st & 1u);
-    x.quads = w.d_quads + (size_t)st * nq + q0;
+    x.quads = skipping ? w.d_quads + (size_t)st * nq : w.d_quads + (size_t)st * nq + q0;
+    x.qord = skipping ? w.d_qord + (size_t)st * nq + q0 : nullptr;
     x.nquads = n;
     x.niw = skipping ? n * w.nchunk : (n * w.nchunk + 1u) / 2u;
     x.upart = w.part((st + 1u) & 1u);
@@ -673,8 +677,10 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   // with the learned skip, each half-list's running visits first
   const uint16_t *const plist = skipping ? w.d_plist_run : w.d_plist;
   mark();
-  if (skipping)
+  if (skipping) {
     launch_order_visits(s, w.d_plist, w.d_plist_run, w.d_skip, np, S, 2u * h);
+    launch_order_quads(s, w.d_quads, w.d_qord, w.d_skip, np, S, h);
+  }
   launch_gram(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
   if (split)
     launch_gram_split(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);

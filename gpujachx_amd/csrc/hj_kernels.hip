@@ -1128,7 +1128,7 @@ void k_stage(StageArgs A)
       const unsigned qi = i / A.nchunk, c = i - qi * A.nchunk;
       // a copy in registers: through a reference into global memory its
       // byte fields would be reloaded after every store of the item
-      const Quad Q = A.quads[qi];
+      const Quad Q = A.quads[A.qord ? A.qord[qi] : qi];
       upgram_item(S.upg, A.G, A.ldG, A.upairs, A.uW, A.uflags, Q, c, A.nchunk, A.CH, A.upart,
                   A.uplan, A.nw);
       __syncthreads();
@@ -1183,6 +1183,68 @@ __global__ __launch_bounds__(256) void k_order_visits(const uint16_t *__restrict
     base += run[255];
     __syncthreads();  // run[] is rewritten by the next chunk
   }
+}
+
+// The fused items of a staged sweep with the learned skip, heavy quads
+// first: per transition t and half, the quads ordered (stably) by the work
+// their items do -- an update per running pair of step t, a Gram per running
+// pair of step t+1 -- so the items that return at once come last and fill
+// the slots the heavy ones leave.  One workgroup of 64 per (transition,
+// half); a half holds at most 64 ... nq/2 quads, looped in chunks of 64.
+__global__ __launch_bounds__(64) void k_order_quads(const Quad *__restrict__ quads,
+                                                    uint16_t *__restrict__ out,
+                                                    const uint8_t *__restrict__ plan,
+                                                    unsigned np, unsigned h)
+{
+  const unsigned t = blockIdx.x >> 1, half = blockIdx.x & 1u, l = threadIdx.x;
+  const unsigned nq = np / 2u, q0 = half ? h : 0u, n = half ? nq - h : h;
+  const Quad *const qs = quads + (size_t)t * nq;
+  const uint8_t *const p0 = plan + (size_t)t * np, *const p1 = plan + (size_t)(t + 1u) * np;
+  uint16_t *const o = out + (size_t)t * nq + q0;
+  // counting sort by weight 4..0 (stable: chunks in order, lanes in order)
+  unsigned start[5];
+  {
+    unsigned cnt[5] = {0u, 0u, 0u, 0u, 0u};
+    for (unsigned c0 = 0u; c0 < n; c0 += 64u) {
+      unsigned w = 5u;
+      if (c0 + l < n) {
+        const Quad Q = qs[q0 + c0 + l];
+        w = 4u - ((p0[Q.p0] != PLAN_SKIP) + (p0[Q.p1] != PLAN_SKIP) + (p1[Q.n0] != PLAN_SKIP) +
+                  (p1[Q.n1] != PLAN_SKIP));
+      }
+#pragma unroll
+      for (unsigned k = 0u; k < 5u; ++k)
+        cnt[k] += (unsigned)__popcll(__ballot(w == k));
+    }
+    unsigned acc = 0u;
+#pragma unroll
+    for (unsigned k = 0u; k < 5u; ++k) {
+      start[k] = acc;
+      acc += cnt[k];
+    }
+  }
+  const unsigned long long below = (l == 0u) ? 0ull : (~0ull >> (64u - l));
+  for (unsigned c0 = 0u; c0 < n; c0 += 64u) {
+    unsigned w = 5u;
+    if (c0 + l < n) {
+      const Quad Q = qs[q0 + c0 + l];
+      w = 4u - ((p0[Q.p0] != PLAN_SKIP) + (p0[Q.p1] != PLAN_SKIP) + (p1[Q.n0] != PLAN_SKIP) +
+                (p1[Q.n1] != PLAN_SKIP));
+    }
+#pragma unroll
+    for (unsigned k = 0u; k < 5u; ++k) {
+      const unsigned long long m = __ballot(w == k);
+      if (w == k)
+        o[start[k] + (unsigned)__popcll(m & below)] = (uint16_t)(q0 + c0 + l);
+      start[k] += (unsigned)__popcll(m);
+    }
+  }
+}
+
+void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, const uint8_t *plan,
+                        unsigned np, unsigned steps, unsigned h)
+{
+  hipLaunchKernelGGL(k_order_quads, dim3(2u * (steps - 1u)), dim3(64), 0, s, quads, out, plan, np, h);
 }
 
 void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,

@@ -151,7 +151,8 @@ struct StageArgs {
   const uint16_t *upairs;  // level-1 pairs of the fused items' step
   const double *uW;        // W of that step
   const unsigned *uflags;  // update flags of that step
-  const Quad *quads;       // first quad of the range
+  const Quad *quads;       // first quad of the range (with qord: of the transition)
+  const uint16_t *qord;    // the range's quads in item order, or null (in index order)
   unsigned nquads;
   unsigned nchunk, CH, nw;
   unsigned niw;            // item workgroups (1..nquads*nchunk; 0 without items)
@@ -172,6 +173,9 @@ struct StageArgs {
 };
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a);
+// quads of a staged sweep's transitions, heaviest items first (learned skip)
+void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, const uint8_t *plan,
+                        unsigned np, unsigned steps, unsigned h);
 // visit lists of a staged sweep, running pairs first (learned skip)
 void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,
                          unsigned np, unsigned steps, unsigned h2);
