@@ -507,7 +507,7 @@ static void issue_step(hipStream_t s, const Work &w, unsigned step, bool sort, u
     launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, skip);
   if (ev) HIP_CALL(hipEventRecord(ev[1], s));
   launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
-              skip, w.d_s0, w.s0steps, nswp, definite, nplus);
+              skip, split, w.d_s0, w.s0steps, nswp, definite, nplus);
   if (ev) HIP_CALL(hipEventRecord(ev[2], s));
   launch_update(s, dG, lddG, nrow, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
   if (ev) HIP_CALL(hipEventRecord(ev[3], s));
@@ -549,8 +549,10 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
         launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, plan);
     }
     if (ev) HIP_CALL(hipEventRecord(ev[1], s));
+    // a split-planned visit holds a split Gram only after a standalone Gram
+    // launch; after a fused transition its Gram is FP64
     launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
-                plan, w.d_s0, w.s0steps, nswp, definite, nplus);
+                plan, split && !have_gram, w.d_s0, w.s0steps, nswp, definite, nplus);
     if (ev) HIP_CALL(hipEventRecord(ev[2], s));
     const bool fused = (st + 1u < w.steps) && w.fuse[st] &&
                        !(steps_left && *steps_left <= 0);
@@ -685,7 +687,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   if (split)
     launch_gram_split(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
   launch_pair(s, sort, w.d_s1, np, w.part(0u), w.nchunk, w.W(0u), w.flags(0u), w.d_ctl, w.d_rec,
-              plan, w.d_s0, w.s0steps, nswp, definite, nplus);
+              plan, split, w.d_s0, w.s0steps, nswp, definite, nplus);
   mark();
   for (unsigned t = 0u; t + 1u < S; ++t) {
     StageArgs x = base;

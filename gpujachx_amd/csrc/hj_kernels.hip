@@ -618,19 +618,30 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
     wo[e] = RA(W, e & 31u, e >> 5);
 }
 
+// The plan a visit runs: a visit planned split whose Gram partials come from
+// the FP64 Gram (a fused transition's, which forms the FP64 Gram for every
+// pair it serves) is an FP64 visit -- it is counted and checked as one.
+__device__ __forceinline__ uint8_t visit_plan(uint8_t plan, bool split_src)
+{
+  return (plan == PLAN_SPLIT && !split_src) ? PLAN_FP64 : plan;
+}
+
+// split_src: the partials of this launch's pairs came from k_gram followed by
+// k_gram_split (a standalone Gram launch), so split-planned pairs hold split
+// Grams
 template <bool SORT>
 __global__ __launch_bounds__(128) void k_pair(const uint16_t *__restrict__ pairs,
                                               const double *__restrict__ part, unsigned nchunk,
                                               double *__restrict__ Wout, unsigned *__restrict__ flags,
                                               RunCtl *__restrict__ ctl, VisitRec *__restrict__ rec,
-                                              const uint8_t *__restrict__ skip,
+                                              const uint8_t *__restrict__ skip, int split_src,
                                               const uint8_t *__restrict__ strat0, unsigned s0steps,
                                               unsigned nswp, int definite, unsigned nplus)
 {
   __shared__ PairLds L;
   __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
   const unsigned tid = threadIdx.x, pi = blockIdx.x;
-  const uint8_t plan = skip ? skip[pi] : PLAN_FP64;
+  const uint8_t plan = visit_plan(skip ? skip[pi] : PLAN_FP64, split_src != 0);
   if (plan == PLAN_SKIP) {
     // skipped visit: no update, the record keeps the last executed visit
     if (tid == 0u) {
@@ -1064,7 +1075,9 @@ void k_flow(FlowArgs A)
                        A.part + (size_t)ring * A.np * A.nchunk * 768u, A.nchunk,
                        A.W + (size_t)ring * A.np * 1024u, A.flags + (size_t)ring * A.np, A.ctl,
                        A.rec + (size_t)st * A.np,
-                       (present && A.plan) ? A.plan[(size_t)st * A.np + pi] : PLAN_FP64, S0,
+                       visit_plan((present && A.plan) ? A.plan[(size_t)st * A.np + pi] : PLAN_FP64,
+                                  st == 0u),
+                       S0,
                        A.s0steps, A.nswp, A.definite, A.nplus);
       __threadfence();
       __syncthreads();
@@ -1102,7 +1115,8 @@ void k_stage(StageArgs A)
     const unsigned half = tid >> 7, ht = tid & 127u, k = 2u * b + half;
     const bool present = k < A.npl;
     const unsigned pi = present ? A.plist[k] : 0u;
-    const uint8_t plan = (present && A.pplan) ? A.pplan[pi] : PLAN_FP64;
+    // every Gram a staged launch's visits read comes from fused items (FP64)
+    const uint8_t plan = visit_plan((present && A.pplan) ? A.pplan[pi] : PLAN_FP64, false);
     pair_visit<SORT>(ht, present, pi, S.pair[half], &S.pair[half ^ 1u].go, A.ppairs, A.ppart,
                      A.nchunk, A.pW, A.pflags, A.ctl, A.prec, plan, S0, A.s0steps, A.nswp,
                      A.definite, A.nplus);
@@ -1415,15 +1429,16 @@ void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint1
 
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
                  unsigned nchunk, double *W, unsigned *flags, RunCtl *ctl, VisitRec *rec,
-                 const uint8_t *skip, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
-                 int definite, unsigned nplus)
+                 const uint8_t *skip, bool split_src, const uint8_t *strat0, unsigned s0steps,
+                 unsigned nswp, int definite, unsigned nplus)
 {
+  const int ss = split_src ? 1 : 0;
   if (sort)
     hipLaunchKernelGGL(k_pair<true>, dim3(np), dim3(128), 0, s, pairs, part, nchunk, W, flags, ctl,
-                       rec, skip, strat0, s0steps, nswp, definite, nplus);
+                       rec, skip, ss, strat0, s0steps, nswp, definite, nplus);
   else
     hipLaunchKernelGGL(k_pair<false>, dim3(np), dim3(128), 0, s, pairs, part, nchunk, W, flags, ctl,
-                       rec, skip, strat0, s0steps, nswp, definite, nplus);
+                       rec, skip, ss, strat0, s0steps, nswp, definite, nplus);
 }
 
 void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsigned chG,

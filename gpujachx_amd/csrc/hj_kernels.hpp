@@ -193,10 +193,12 @@ void launch_flow_reset(hipStream_t s, unsigned *cnt, size_t n);
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
                        unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);
+// split_src: the partials come from a standalone Gram launch, so pairs
+// planned split hold split Grams (else every pair is an FP64 visit)
 void launch_pair(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np, const double *part,
                  unsigned nchunk, double *W, unsigned *flags, RunCtl *ctl, VisitRec *rec,
-                 const uint8_t *skip, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
-                 int definite, unsigned nplus);
+                 const uint8_t *skip, bool split_src, const uint8_t *strat0, unsigned s0steps,
+                 unsigned nswp, int definite, unsigned nplus);
 void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsigned chG,
                    double *V, unsigned ldV, unsigned nrowV, unsigned chV, const uint16_t *pairs,
                    unsigned np, const double *W, const unsigned *flags);
@@ -206,6 +208,7 @@ void launch_initV(hipStream_t s, double *V, unsigned ifc, unsigned nRank, unsign
 void launch_sweep_begin(hipStream_t s, RunCtl *ctl);
 void launch_dbg_scalar(hipStream_t s, int kind, int n, const double *a, const double *b,
                        const double *c, double *oc, double *ot, int *ob);
+
 
 }  // namespace hj
 

@@ -46,3 +46,21 @@ def graded_input(m, n, seed, lo=-4.0, hi=4.0):
 def strat0_cycwor():
     import gpujachx_amd as g
     return g.strat_generate("cycwor", 32).astype(np.uint8)
+
+
+def assert_per_value(fast, plain, spread, ctx="", factor=10.0):
+    """Per-value agreement of a fast form's spectrum `fast` with the plain
+    FP64 run's `plain` (sigma, or D = j sigma^2): every sorted value within
+    `factor` times the largest relative spread of another FP64 run `spread`
+    of the same matrix (1e-12 at least), the values of at least a tenth of
+    the largest to 1e-11.  A wrong small value fails here even where a
+    normwise comparison cannot see it."""
+    from gpujachx_amd.accuracy import rel_diffs
+    rel_f, ref = rel_diffs(fast, plain)
+    rel_s, _ = rel_diffs(spread, plain)
+    tol = max(factor * rel_s.max(), 1e-12)
+    assert rel_f.max() <= tol, "%s: per-value max %.3g > %.3g (FP64 spread %.3g)" % (
+        ctx, rel_f.max(), tol, rel_s.max())
+    big = ref >= 0.1 * ref.max()
+    assert np.all(rel_f[big] <= 1e-11), "%s: large values %.3g" % (ctx, rel_f[big].max())
+    return rel_f.max(), rel_s.max()

@@ -7,7 +7,7 @@ import gpujachx_amd as g
 import gpujachx_amd.learned as lr
 import oracle as o
 from oracle import mlp
-from conftest import graded_input, svd_input
+from conftest import assert_per_value, graded_input, svd_input
 
 pytestmark = pytest.mark.gpu
 
@@ -92,11 +92,13 @@ def _use_model(p, thr=0.05, max_skips=2, min_frac=0.0, min_propose=0.0):
     g.set_form(g.FORM_LEARNED_SKIP)
 
 
-def _plain(alg, G, nplus):
+def _plain(alg, G, nplus, snp1="cycwor"):
     g.set_form(g.FORM_PLAIN)
     n = G.shape[1]
+    g.init_strats_builtin("cycwor", 32, snp1, n // 16)
+    r = g.hypJacL2(alg, G, nplus)
     g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
-    return g.hypJacL2(alg, G, nplus)
+    return r
 
 
 @pytest.mark.parametrize("alg,kind", [(9, "svd"), (11, "hyp"), (3, "svd")])
@@ -153,6 +155,7 @@ def test_learned_model_file_converges():
     if not os.path.exists(lr.DEFAULT_MODEL):
         pytest.skip("no trained model file")
     G = svd_input(1024, 1024, 10.0 ** (-10.0 * np.random.default_rng(3).uniform(size=1024)), seed=31)
+    rr = _plain(9, G, 1024, snp1="rowcyc")
     rp = _plain(9, G, 1024)
     lr.enable()
     g.lib().hj_set_skip_min_visits(0)
@@ -165,6 +168,7 @@ def test_learned_model_file_converges():
     assert rb[-1] == 0 and sk[-1] == 0
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()
+    assert_per_value(rl.D, rp.D, rr.D, "packaged model N=1024")
 
 
 def test_skip_policy_gates():
@@ -217,6 +221,7 @@ def test_learned_gain_at_4096():
     G = svd_input(n, n, s, seed=33)
     old = g.set_max_sweeps(80)
     try:
+        rr = _plain(9, G, n, snp1="rowcyc")
         t0 = time.perf_counter()
         rp = _plain(9, G, n)
         tp = time.perf_counter() - t0
@@ -236,3 +241,5 @@ def test_learned_gain_at_4096():
     assert vl <= 0.9 * vp, (vl, vp)
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()
+    print("  per value vs plain %.3g, FP64 spread %.3g" % assert_per_value(rl.D, rp.D, rr.D,
+                                                                           "N=4096"))

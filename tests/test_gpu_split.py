@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import gpujachx_amd as g
-from conftest import graded_input, svd_input
+from conftest import assert_per_value, graded_input, svd_input
 
 pytestmark = pytest.mark.gpu
 
@@ -49,9 +49,9 @@ def test_split_gram_relative_error(nrow, lo, hi):
     assert (np.abs(A64 - ref) / np.outer(d, d)).max() < 1e-13
 
 
-def _run(form, alg, G, nplus, tau=1e-3):
+def _run(form, alg, G, nplus, tau=1e-3, snp1="cycwor"):
     n = G.shape[1]
-    g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+    g.init_strats_builtin("cycwor", 32, snp1, n // 16)
     g.set_form(form)
     g.set_split_tau(tau)
     try:
@@ -73,11 +73,13 @@ def test_split_form_matches_plain(alg, kind):
         G = graded_input(512, n, seed=42)
         nplus = 100
     rp, _, _, _ = _run(g.FORM_PLAIN, alg, G, nplus)
+    rr, _, _, _ = _run(g.FORM_PLAIN, alg, G, nplus, snp1="rowcyc")
     rs, sp, tot, rb = _run(g.FORM_SPLIT_GRAM, alg, G, nplus)
     assert tot > 0 and sp[0] > 0
     assert sp[-1] == 0 and rb[-1] == 0          # ends on an all-FP64 sweep
     a, b = np.sort(rp.sigma), np.sort(rs.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()
+    assert_per_value(rs.D, rp.D, rr.D, "split alg %d %s" % (alg, kind))
     if rs.V is not None:
         J = np.diag(np.r_[np.ones(nplus), -np.ones(n - nplus)])
         assert np.abs(rs.V.T @ J @ rs.V - J).max() < 1e-11 * max(1.0, np.abs(rs.V).max() ** 2)
@@ -108,6 +110,7 @@ def test_learned_plus_split_matches_plain():
     old = g.set_max_sweeps(80)
     try:
         rp, _, _, rbp = _run(g.FORM_PLAIN, 9, G, 1024)
+        rr, _, _, _ = _run(g.FORM_PLAIN, 9, G, 1024, snp1="rowcyc")
         g.init_strats_builtin("cycwor", 32, "cycwor", 64)
         lr.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
         g.lib().hj_set_skip_min_visits(0)
@@ -123,3 +126,4 @@ def test_learned_plus_split_matches_plain():
     assert rb[-1] == 0 and sp[-1] == 0 and sk[-1] == 0
     a, b = np.sort(rp.sigma), np.sort(rl.sigma)
     assert np.abs(a - b).max() <= 1e-13 * a.max()
+    assert_per_value(rl.D, rp.D, rr.D, "learned+split N=1024")

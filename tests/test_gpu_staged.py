@@ -98,10 +98,16 @@ def test_staged_not_used_where_it_does_not_apply():
         g.set_staged(old)
 
 
-@pytest.mark.parametrize("form", ["learned+split", "split"])
-def test_staged_forms_equal_step_by_step(form):
-    # skipped and split visits inside staged launches: the same bits as the
-    # step-by-step launches of the same form
+@pytest.mark.parametrize("form,alg,kind,snp1", [
+    ("learned+split", 9, "svd", "cycwor"), ("split", 9, "svd", "cycwor"),
+    # the V update of the staged launches, a mixed J (NPLUS = N/2) and a
+    # strategy whose cross transitions differ from cycwor's
+    ("learned+split", 11, "svd", "cycwor"), ("learned+split", 9, "hyp", "cycwor"),
+    ("learned+split", 11, "hyp", "rowcyc"), ("learned+split", 9, "svd", "rowcyc")])
+def test_staged_forms_equal_step_by_step(form, alg, kind, snp1):
+    # skipped and split visits inside staged launches (with the learned
+    # skip: the sweep's visit lists and quads reordered by the plan): the
+    # same bits as the step-by-step launches of the same form
     import os
     import gpujachx_amd.learned as learned
     if "learned" in form and not os.path.exists(learned.DEFAULT_MODEL):
@@ -109,31 +115,37 @@ def test_staged_forms_equal_step_by_step(form):
     n = 1024
     s = 10.0 ** (-10.0 * np.random.default_rng(3).uniform(size=n))
     G = svd_input(n, n, s, seed=31)
+    nplus = n if kind == "svd" else n // 2
     outs = []
     for staged in (True, False):
         if "learned" in form:
             learned.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
             g.lib().hj_set_skip_min_visits(0)
+            g.lib().hj_set_skip_min_propose(0.0)
         else:
             g.set_form(g.FORM_SPLIT_GRAM)
         try:
-            r, used = run(9, G, n, staged)
+            r, used = run(alg, G, nplus, staged, snp1)
             outs.append((r, used, g.last_skips()[1], g.last_splits()[1]))
         finally:
             g.set_form(g.FORM_PLAIN)
             g.lib().hj_set_skip_min_visits(32000)
+            g.lib().hj_set_skip_min_propose(0.1)
     (a, ua, ska, spa), (b, ub, skb, spb) = outs
     assert ua and not ub
     assert (ska, spa) == (skb, spb)
     assert spa > 0
-    assert_same(a, b, "%s (skipped %d, split %d)" % (form, ska, spa))
+    if "learned" in form:
+        assert ska > 0
+    assert_same(a, b, "%s alg %d %s %s (skipped %d, split %d)" % (form, alg, kind, snp1, ska, spa))
 
 
-@pytest.mark.parametrize("form,sweeps", [("plain", 1), ("learned+split", 3)])
+@pytest.mark.parametrize("form,sweeps", [("plain", 1), ("learned+split", 12)])
 def test_staged_full_size(form, sweeps):
     # N = 8192, the bench's size and launch plan (256 pairs per step, the
     # cross transitions where the strategy puts them): the first sweeps of
-    # the staged and the step-by-step launches, every bit of G and D equal
+    # the staged and the step-by-step launches, every bit of G and D equal;
+    # the learned form runs into the sweeps where it skips
     import ctypes
     import os
     import torch
@@ -171,5 +183,7 @@ def test_staged_full_size(form, sweeps):
     (Wa, Da, *ra), (Wb, Db, *rb) = outs
     assert ra[3] and not rb[3]
     assert ra[:3] == rb[:3] and ra[4] == rb[4], (ra, rb)
+    if "learned" in form:
+        assert ra[4] > 0, ra
     assert torch.equal(Da, Db)
     assert torch.equal(Wa, Wb)
