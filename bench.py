@@ -109,6 +109,8 @@ def cpu_baseline(n_target, seed, budget_s=150.0):
         except subprocess.TimeoutExpired:
             pts.append({"n": n, "seconds": None, "timed_out_after_s": left})
         left -= time.time() - t0
+        print("bench: dgesvj n=%d %s" % (n, pts[-1] if pts else "failed"), file=sys.stderr,
+              flush=True)
     done = [p for p in pts if p.get("seconds")]
     if not done:
         return None

@@ -1,25 +1,38 @@
-// hj_qr.hip -- the QR factorization path of a block-pair visit and Drmac's
-// sorted level-0 for gfx950 (the reference's build options USE_QR and
-// USE_DRMAC, share/device_code_common.hpp:78-85,140-144).
+// hj_qr.hip -- the QR factorization of a block-pair visit and Drmac's sorted
+// level-0 (the reference's build options USE_QR and USE_DRMAC,
+// share/device_code_common.hpp:78-85,140-144), one wave per block pair.
 //
-// One workgroup of 512 lanes per block pair, in the reference's layout:
-// lane x = tid % 32 is a row of the 32-row working blocks, y = tid / 32 a
-// column pair (columns y and y + 16); each half of a 64-lane wave is one
-// y-group, so the 32-element butterflies (16,8,4,2,1) stay inside a
-// half-wave (ds_swizzle).  The visit:
-//   factorize: R of the nrow x 32 column pair [G_i G_j] by 32-row panels in
-//     order -- Householder QR of the first panel (dQR32,
-//     share/device_code_common_Kepler.hpp:326-371), then per further panel
-//     its Householder QR merged into R by a wavefront of Givens rotations
-//     (dPeelOff / dFactorize, share/device_code_common_QR.hpp:4-94);
-//   level-0: the reference's inner sweeps on R (8 bits of the dot products
-//     identical to the Cholesky path's), or for sorted definite blocks with
-//     DRMAC, Drmac's sweeps with overflow-safe norms and cosines
-//     (share/device_code_cdsort_Drmac.hpp:4-316);
-//   W to HBM; the block update is k_update's.
-// This path is latency-bound (about 190 workgroup barriers per 32 rows of
-// the pair); it exists, as in the reference, as an accuracy option for
-// extremely graded column pairs, not for speed.
+// Layout.  A 64-lane wave owns one pair.  Its LDS holds two 32x32 blocks,
+// R (the running triangular factor) and P (the panel being reduced; W once
+// the factorization is done), column-major with leading dimension 33: the
+// L0A layout of the Cholesky path, so that path's wave_level0 runs on them
+// unchanged.  Lanes form 16 quads; lane r of quad q holds rows r, r+4, ...,
+// r+28 of whatever column the quad works on, so a 32-row reduction is the
+// butterfly 16,8,4,2,1 with levels 16, 8, 4 inside the lane and 2, 1 by
+// quad DPP (sum4x8) -- the bits of the reference's 32-lane dSum32.  A single
+// wave needs no workgroup barrier: its LDS accesses complete in program
+// order, and wave_lds_sync() only keeps the compiler from moving them from
+// one phase into the next.
+//
+// The visit:
+//  * factorization by 32-row panels in order (dFactorize,
+//    share/device_code_common_QR.hpp:68-92): Householder QR of the first
+//    panel into R; for each further panel, its Householder QR into P, then
+//    R <- R of [R; P] by a wavefront of Givens rotations (dPeelOff, :4-66).
+//    A panel's global loads are issued before the previous panel's
+//    reduction, so they arrive while it runs.
+//      Householder step k (dQR32, share/device_code_common_Kepler.hpp:326-371):
+//      every quad forms the overflow-safe norm of column k itself (no
+//      broadcast), then quad q reflects columns k+1+q and k+17+q.
+//      Givens step k: lanes 0..31 each form the rotation of one row x >= k
+//      (with row x-k of P) into LDS; quad q then rotates columns k+q and
+//      k+16+q.
+//  * level-0: Drmac's sweeps for sorted definite blocks when asked
+//    (share/device_code_cdsort_Drmac.hpp:4-316), quad q taking pivot pair q
+//    of each step; otherwise the Cholesky path's wave_level0.
+//  * counters, the update flag and W to HBM; the block update is k_update's.
+// This path is latency-bound; like the reference's, it is an accuracy option
+// for extremely graded column pairs, not a fast path.
 #include <climits>
 #include <cfloat>
 
@@ -29,556 +42,524 @@
 namespace hj {
 namespace {
 
-constexpr unsigned QLD = 33u;  // padded leading dimension of the LDS blocks
-#define QA(A, r, c) (A)[(c) * QLD + (r)]
+// Magnitude limits of the overflow- and underflow-safe sums of squares
+// (share/HypJacL.hpp:53-86).  Squares of entries in [SSQ_SMALL, SSQ_BIG]
+// and 32-term sums of them stay normal and finite.
+constexpr double SSQ_SMALL = 0x1p-511;              // sqrt of the smallest normal
+constexpr double SSQ_BIG = 0x1.6a09e667f3bcap+509;  // a little below sqrt(DBL_MAX / 32)
+constexpr double PROD_SMALL = 0x1p-970;             // smallest normal / epsilon
+constexpr double COT_LIMIT = 0x1.fffffffffffffp+1020;  // DBL_MAX / 8
 
-// MU_, NU_, MU_EPS, NU_8 of share/HypJacL.hpp:53-86
-constexpr double QR_MU = 1.4916681462400414314624091158735E-154;
-constexpr double QR_NU = 2.370187977027293260705088939015E+153;
-constexpr double QR_MU_EPS = 1.0020841800044863889980540256751E-292;
-constexpr double QR_NU_8 = 2.24711641857789464E+307;
+// ---------------------------------------------------------------------------
+// Reductions over the 32 rows of one column held by a quad, v[j] = row r + 4j.
 
-// max / min of |x| over the 32 lanes of a half-wave (the butterfly of
-// dMax32 / dMin32, share/device_code_common_Kepler.hpp:76-164)
-__device__ __forceinline__ double max32(double x)
+__device__ __forceinline__ double quad_max_abs(const double (&v)[8])
 {
-  double v = fabs(x);
-  v = fmax(v, swz_xor<16>(v));
-  v = fmax(v, swz_xor<8>(v));
-  v = fmax(v, swz_xor<4>(v));
-  v = fmax(v, swz_xor<2>(v));
-  v = fmax(v, swz_xor<1>(v));
-  return v;
+  double m = fabs(v[0]);
+#pragma unroll
+  for (unsigned j = 1u; j < 8u; ++j) m = fmax(m, fabs(v[j]));
+  m = fmax(m, lane_xor2(m));
+  return fmax(m, lane_xor1(m));
 }
 
-__device__ __forceinline__ double min32_nz(double x)
+// smallest nonzero magnitude (DBL_MAX when every entry is zero)
+__device__ __forceinline__ double quad_min_abs_nz(const double (&v)[8])
 {
-  double v = (x == 0.0) ? DBL_MAX : fabs(x);
-  v = fmin(v, swz_xor<16>(v));
-  v = fmin(v, swz_xor<8>(v));
-  v = fmin(v, swz_xor<4>(v));
-  v = fmin(v, swz_xor<2>(v));
-  v = fmin(v, swz_xor<1>(v));
-  return v;
+  double m = DBL_MAX;
+#pragma unroll
+  for (unsigned j = 0u; j < 8u; ++j) m = fmin(m, (v[j] == 0.0) ? DBL_MAX : fabs(v[j]));
+  m = fmin(m, lane_xor2(m));
+  return fmin(m, lane_xor1(m));
 }
 
-// share/device_code_common_scale.hpp:4-56
-__device__ __forceinline__ int find_scl(double x, double t)
+// The power of two that moves x next to the limit t: for x <= t the least s
+// with x 2^s >= t, for x > t the greatest s with x 2^s <= t.
+__device__ __forceinline__ int scale_to(double x, double t)
 {
-  int x_n, t_n;
-  const double x_m = frexp(x, &x_n);
-  const double t_m = frexp(t, &t_n);
-  return (x <= t) ? ((t_n - x_n) + (x_m < t_m)) : ((t_n - x_n) - (x_m > t_m));
+  int ex, et;
+  const double mx = frexp(x, &ex), mt = frexp(t, &et);
+  const int s = et - ex;
+  return (x <= t) ? s + (mx < mt ? 1 : 0) : s - (mx > mt ? 1 : 0);
 }
 
-__device__ __forceinline__ void normalize2(int &scl, double &ssq)
+// A scaled value f 2^e with f brought to an even exponent's power of two
+// removed (f in [1, 4)), so that the square root rescales exactly.
+__device__ __forceinline__ void even_normalize(int &e, double &f)
 {
-  int s = ilogb(ssq);
-  s = s + (s & 1);
-  scl += s;
-  ssq = scalbn(ssq, -s);
+  int s = ilogb(f);
+  s += s & 1;
+  e += s;
+  f = scalbn(f, -s);
 }
 
-__device__ __forceinline__ void sum_scl(int s1, int s2, double x1, double x2, int &s, double &x)
+// (e, f) = f1 2^e1 + f2 2^e2: the smaller term, by exponent then fraction,
+// rescaled to the larger one's exponent.
+__device__ __forceinline__ void add_scaled(int e1, double f1, int e2, double f2, int &e, double &f)
 {
-  const int cmp = (s1 < s2) ? -1 : ((s2 < s1) ? 1 : ((x1 < x2) ? -1 : ((x2 < x1) ? 1 : 0)));
-  if (cmp <= 0) {
-    s = s2;
-    x = scalbn(x1, s1 - s2) + x2;
+  const bool first_smaller = (e1 != e2) ? (e1 < e2) : !(f2 < f1);
+  if (first_smaller) {
+    e = e2;
+    f = scalbn(f1, e1 - e2) + f2;
   }
   else {
-    s = s1;
-    x = scalbn(x2, s2 - s1) + x1;
+    e = e1;
+    f = scalbn(f2, e2 - e1) + f1;
   }
 }
 
-// band 2: |x| > NU (scaled), 1: MU <= |x| <= NU, 0: |x| < MU (scaled)
-__device__ __forceinline__ double ssq_band(double x, int band, int scl)
+// Sum of squares of the entries in one magnitude band, each scaled by 2^e
+// first (outer bands) or taken as is (middle band).
+enum Band { BAND_LOW = 0, BAND_MID = 1, BAND_HIGH = 2 };
+
+__device__ __forceinline__ double band_ssq(const double (&v)[8], Band b, int e)
 {
-  const double x_ = fabs(x);
-  double y;
-  if (band == 2) y = (x_ > QR_NU) ? scalbn(x_, scl) : 0.0;
-  else if (band == 1) y = ((x_ >= QR_MU) && (x_ <= QR_NU)) ? x_ : 0.0;
-  else y = (x_ < QR_MU) ? scalbn(x_, scl) : 0.0;
-  return sum32(y * y);
+  double sq[8];
+#pragma unroll
+  for (unsigned j = 0u; j < 8u; ++j) {
+    const double a = fabs(v[j]);
+    double y;
+    if (b == BAND_HIGH) y = (a > SSQ_BIG) ? scalbn(a, e) : 0.0;
+    else if (b == BAND_MID) y = ((a >= SSQ_SMALL) && (a <= SSQ_BIG)) ? a : 0.0;
+    else y = (a < SSQ_SMALL) ? scalbn(a, e) : 0.0;
+    sq[j] = y * y;
+  }
+  return sum4x8(sq);
 }
 
-// dSSQ32, share/device_code_common_Kepler.hpp:203-260 (uniform per half-wave)
-__device__ int ssq32(double &ssq, double x)
+// The sum of squares of the column as f 2^e (returned), without overflow
+// or harmful underflow: entries above SSQ_BIG or below SSQ_SMALL are summed
+// in a band of their own, scaled toward the limit, and the band sums are
+// combined by add_scaled (dSSQ32, share/device_code_common_Kepler.hpp:203-260).
+// e is INT_MIN for a zero column.
+__device__ int quad_ssq(const double (&v)[8], double &f)
 {
-  const double M = max32(x);
-  if (M == 0.0) {
-    ssq = 0.0;
+  const double hi = quad_max_abs(v);
+  if (hi == 0.0) {
+    f = 0.0;
     return INT_MIN;
   }
-  const double m = min32_nz(x);
-  const bool lo = (m < QR_MU), hi = (M > QR_NU);
-  int scl;
-  if (hi) {
-    int sc2 = find_scl(M, QR_NU);
-    double sq2 = ssq_band(x, 2, sc2);
-    sc2 *= -2;
-    if (m > QR_NU) {
-      scl = sc2;
-      ssq = sq2;
+  const double lo = quad_min_abs_nz(v);
+  if (hi > SSQ_BIG) {
+    int eh = scale_to(hi, SSQ_BIG);
+    double fh = band_ssq(v, BAND_HIGH, eh);
+    eh = -2 * eh;
+    if (lo > SSQ_BIG) {
+      f = fh;
+      return eh;
     }
-    else {
-      int sc1 = 0;
-      double sq1 = ssq_band(x, 1, 0);
-      if (sq1 > 0.0) {
-        normalize2(sc1, sq1);
-        normalize2(sc2, sq2);
-        sum_scl(sc1, sc2, sq1, sq2, scl, ssq);
-      }
-      else {
-        scl = sc2;
-        ssq = sq2;
-      }
+    int em = 0;
+    double fm = band_ssq(v, BAND_MID, 0);
+    if (!(fm > 0.0)) {
+      f = fh;
+      return eh;
     }
+    even_normalize(em, fm);
+    even_normalize(eh, fh);
+    int e;
+    add_scaled(em, fm, eh, fh, e, f);
+    return e;
   }
-  else if (lo) {
-    int sc0 = find_scl(m, QR_MU);
-    double sq0 = ssq_band(x, 0, sc0);
-    sc0 *= -2;
-    if (M < QR_MU) {
-      scl = sc0;
-      ssq = sq0;
+  if (lo < SSQ_SMALL) {
+    int el = scale_to(lo, SSQ_SMALL);
+    double fl = band_ssq(v, BAND_LOW, el);
+    el = -2 * el;
+    if (hi < SSQ_SMALL) {
+      f = fl;
+      return el;
     }
-    else {
-      int sc1 = 0;
-      double sq1 = ssq_band(x, 1, 0);
-      if (sq1 > 0.0) {
-        normalize2(sc1, sq1);
-        normalize2(sc0, sq0);
-        sum_scl(sc0, sc1, sq0, sq1, scl, ssq);
-      }
-      else {
-        scl = sc0;
-        ssq = sq0;
-      }
+    int em = 0;
+    double fm = band_ssq(v, BAND_MID, 0);
+    if (!(fm > 0.0)) {
+      f = fl;
+      return el;
     }
+    even_normalize(em, fm);
+    even_normalize(el, fl);
+    int e;
+    add_scaled(el, fl, em, fm, e, f);
+    return e;
   }
-  else {
-    scl = 0;
-    ssq = sum32(x * x);
-  }
-  return scl;
+  double sq[8];
+#pragma unroll
+  for (unsigned j = 0u; j < 8u; ++j) sq[j] = v[j] * v[j];
+  f = sum4x8(sq);
+  return 0;
 }
 
-// dNRM2_32, share/device_code_common_Kepler.hpp:269-275
-__device__ __forceinline__ double nrm2_32(double x)
+// ||column||_2 (dNRM2_32, share/device_code_common_Kepler.hpp:269-275)
+__device__ __forceinline__ double quad_nrm2(const double (&v)[8])
 {
-  double ssq;
-  const int scl = ssq32(ssq, x) / 2;
-  return scalbn(sqrt(ssq), scl);
+  double f;
+  const int e = quad_ssq(v, f);
+  return scalbn(sqrt(f), e / 2);
 }
 
-// dCOSA_ for Ap >= Aq, share/device_code_common_Kepler.hpp:277-303
-__device__ __forceinline__ double cosa_(double Ap, double Aq, double x, double y)
+// x^T y / (nx ny) for nx >= ny, the division placed so that neither the
+// product nor the quotient leaves the normal range
+// (dCOSA_, share/device_code_common_Kepler.hpp:277-303)
+__device__ __forceinline__ double quad_cos_ordered(double nx, double ny, const double (&x)[8],
+                                                   const double (&y)[8])
 {
-  bool big, small;
-  if (Aq >= 1.0) {
-    big = ((Ap * Aq) >= DBL_MAX);
-    small = false;
+  const bool huge = (ny >= 1.0) && ((nx * ny) >= DBL_MAX);
+  const bool tiny = !(ny >= 1.0) && ((nx * ny) <= PROD_SMALL);
+  double z[8];
+  if (huge) {
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j)
+      z[j] = (fabs(x[j]) >= fabs(y[j])) ? ((x[j] / nx) * y[j]) : ((y[j] / nx) * x[j]);
+    return sum4x8(z) / ny;
   }
-  else {
-    big = false;
-    small = ((Ap * Aq) <= QR_MU_EPS);
+  if (tiny) {
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j)
+      z[j] = (fabs(x[j]) >= fabs(y[j])) ? (x[j] * (y[j] / ny)) : (y[j] * (x[j] / ny));
+    return sum4x8(z) / nx;
   }
-  if (big) {
-    const double z = (fabs(x) >= fabs(y)) ? ((x / Ap) * y) : ((y / Ap) * x);
-    return sum32(z) / Aq;
-  }
-  if (small) {
-    const double z = (fabs(x) >= fabs(y)) ? (x * (y / Aq)) : (y * (x / Aq));
-    return sum32(z) / Ap;
-  }
-  return (sum32(x * y) / Aq) / Ap;
+#pragma unroll
+  for (unsigned j = 0u; j < 8u; ++j) z[j] = x[j] * y[j];
+  return (sum4x8(z) / ny) / nx;
 }
 
-__device__ __forceinline__ double cosa(double Ap, double Aq, double x, double y)
+__device__ __forceinline__ double quad_cos(double np, double nq, const double (&p)[8],
+                                           const double (&q)[8])
 {
-  return (Ap >= Aq) ? cosa_(Ap, Aq, x, y) : cosa_(Aq, Ap, y, x);
+  return (np >= nq) ? quad_cos_ordered(np, nq, p, q) : quad_cos_ordered(nq, np, q, p);
 }
 
-// dGivens, share/device_code_common_rotate.hpp:120-155
+// ---------------------------------------------------------------------------
+// Factorization.
+
+// [c s; -s c] [f; g] = [r; 0] with r >= 0, r formed as the larger magnitude
+// times sqrt(1 + (smaller / larger)^2) (dGivens,
+// share/device_code_common_rotate.hpp:120-155)
 __device__ __forceinline__ void givens(double f, double g, double &c, double &s, double &r)
 {
-  const double f_ = fabs(f), g_ = fabs(g);
-  if (f_ >= g_) {
-    if (g_ == 0.0) {
-      c = copysign(1.0, f);
-      s = 0.0;
-      r = f_;
-    }
-    else {
-      const double g_f = g_ / f_;
-      r = f_ * sqrt(__fma_rn(g_f, g_f, 1.0));
-      c = f / r;
-      s = g / r;
-    }
+  const double af = fabs(f), ag = fabs(g);
+  const bool fbig = af >= ag;
+  const double big = fbig ? af : ag, small = fbig ? ag : af;
+  if (small == 0.0) {
+    r = big;
+    c = fbig ? copysign(1.0, f) : 0.0;
+    s = fbig ? 0.0 : copysign(1.0, g);
+    return;
   }
-  else {
-    if (f_ == 0.0) {
-      c = 0.0;
-      s = copysign(1.0, g);
-      r = g_;
-    }
-    else {
-      const double f_g = f_ / g_;
-      r = g_ * sqrt(__fma_rn(f_g, f_g, 1.0));
-      c = f / r;
-      s = g / r;
-    }
-  }
+  const double q = small / big;
+  r = big * sqrt(__fma_rn(q, q, 1.0));
+  c = f / r;
+  s = g / r;
 }
 
-// one Householder application to column col at step k (reflector v with
-// v_k = 1, v_x = A(x,k) / (alpha + beta) below, scale tau)
-__device__ __forceinline__ void hh_apply(double *A, unsigned x, unsigned k, unsigned col,
-                                         double Axk_, double alpha_beta, double tau)
+// Householder QR of the 32x32 block A in place: for k = 0..30 the reflector
+// I - tau v v^T with v_k = 1, v_x = A(x,k) / (alpha + beta) below the
+// diagonal, beta = sign(alpha) ||A(k:31,k)||, applied to columns k+1..31 as
+// A(:,c) += (-(tau v^T A(:,c))) v; column k becomes (-beta, 0, ..., 0).
+__device__ void householder32(double *A, unsigned q, unsigned r)
 {
-  const double Axy = QA(A, x, col);
-  const double Vxk = (x == k) ? 1.0 : ((x > k) ? (Axk_ / alpha_beta) : 0.0);
-  const double dp = sum32(Vxk * Axy);
-  const double tdp = -(tau * dp);
-  QA(A, x, col) = __fma_rn(tdp, Vxk, Axy);
-}
-
-// dQR32: Householder QR of the 32x32 block A in LDS, in place
-__device__ void qr32(double *A, unsigned x, unsigned y0)
-{
-  const unsigned y1 = y0 + 16u;
 #pragma unroll 1
   for (unsigned k = 0u; k < 31u; ++k) {
-    double nrm2 = 0.0, Axk = 0.0, Axk_ = 0.0, beta = 0.0, alpha_beta = 0.0, tau = 0.0;
-    const unsigned my0 = k + y0;
-    if (my0 < 32u) {
-      Axk = QA(A, x, k);
-      Axk_ = (x >= k) ? Axk : 0.0;
-      nrm2 = nrm2_32(Axk_);
-      if (nrm2 > 0.0) {
-        const double alpha = QA(A, k, k);
-        beta = copysign(nrm2, alpha);
-        alpha_beta = alpha + beta;
-        tau = alpha_beta / beta;
+    double v[8];
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      const unsigned x = r + 4u * j;
+      const double a = L0A(A, x, k);
+      v[j] = (x >= k) ? a : 0.0;
+    }
+    const double nrm = quad_nrm2(v);
+    if (!(nrm > 0.0))
+      continue;  // the same in every lane: every quad holds column k
+    const double alpha = L0A(A, k, k);
+    // column k has been read by every lane before any lane writes it
+    wave_lds_sync();
+    const double beta = copysign(nrm, alpha), ab = alpha + beta, tau = ab / beta;
+#pragma unroll
+    for (unsigned j = 0u; j < 8u; ++j) {
+      const unsigned x = r + 4u * j;
+      v[j] = (x == k) ? 1.0 : ((x > k) ? v[j] / ab : 0.0);
+    }
+    for (unsigned c = k + 1u + q; c < 32u; c += 16u) {
+      double a[8], pr[8];
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j) {
+        a[j] = L0A(A, r + 4u * j, c);
+        pr[j] = v[j] * a[j];
+      }
+      const double tdp = -(tau * sum4x8(pr));
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j)
+        L0A(A, r + 4u * j, c) = __fma_rn(tdp, v[j], a[j]);
+    }
+    if (q == 15u) {
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j) {
+        const unsigned x = r + 4u * j;
+        if (x >= k) L0A(A, x, k) = (x == k) ? -beta : 0.0;
       }
     }
-    __syncthreads();
-    if (nrm2 > 0.0) {
-      if (my0 == k)
-        QA(A, x, k) = (x == k) ? -beta : ((x > k) ? 0.0 : Axk);
-      else
-        hh_apply(A, x, k, my0, Axk_, alpha_beta, tau);
-      const unsigned my1 = k + y1;
-      if (my1 < 32u)
-        hh_apply(A, x, k, my1, Axk_, alpha_beta, tau);
-    }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
-// dPeelOff: R0 <- R of [R0; R1] by Givens rotations, row x of R0 meeting row
-// x - k of R1 at step k
-__device__ void peeloff(double *R0, double *R1, unsigned x, unsigned y0)
+// R0 <- the R factor of [R0; R1] for upper triangular R0, R1: at step k,
+// row x >= k of R0 meets row x-k of R1 in the rotation that zeroes R1(x-k, x)
+// against R0(x, x), applied to columns x..31 (products and sums rounded
+// separately).  gv: 96 doubles of LDS for the step's c, s, r.
+__device__ void merge_rows(double *R0, double *R1, double *gv, unsigned l, unsigned q, unsigned r)
 {
-  const unsigned y1 = y0 + 16u;
 #pragma unroll 1
   for (unsigned k = 0u; k < 32u; ++k) {
-    const unsigned my0 = k + y0, my1 = k + y1;
-    const bool active = (x >= k) && (x <= my1);
-    const unsigned xk = x - k;
-    double c = 0.0, s = 0.0, r = 0.0, a_ = 0.0, b_ = 0.0;
-    bool store = false;
-    if ((my0 < 32u) && active) {
-      givens(QA(R0, x, x), QA(R1, xk, x), c, s, r);
-      if (x < my0) {
-        const double a = QA(R0, x, my0), b = QA(R1, xk, my0);
-        const double ca = c * a, sb = s * b, cb = c * b, sa = s * a;
-        a_ = ca + sb;
-        b_ = cb - sa;
-        store = true;
-      }
-      else if (x == my0) {
-        a_ = r;
-        b_ = 0.0;
-        store = true;
-      }
+    if ((l < 32u) && (l >= k)) {
+      double c, s, rr;
+      givens(L0A(R0, l, l), L0A(R1, l - k, l), c, s, rr);
+      gv[l] = c;
+      gv[32u + l] = s;
+      gv[64u + l] = rr;
     }
-    __syncthreads();
-    if (store) {
-      QA(R0, x, my0) = a_;
-      QA(R1, xk, my0) = b_;
-      store = false;
-    }
-    __syncthreads();
-    if ((my1 < 32u) && active) {
-      if (x < my1) {
-        const double a = QA(R0, x, my1), b = QA(R1, xk, my1);
-        const double ca = c * a, sb = s * b, cb = c * b, sa = s * a;
-        a_ = ca + sb;
-        b_ = cb - sa;
-        store = true;
-      }
-      else if (x == my1) {
-        a_ = r;
-        b_ = 0.0;
-        store = true;
-      }
-    }
-    __syncthreads();
-    if (store) {
-      QA(R0, x, my1) = a_;
-      QA(R1, xk, my1) = b_;
-    }
-    __syncthreads();
-  }
-}
-
-// sum over the 16 y-groups of a per-group count (lane x == 0 of each group
-// holds it); every lane gets the total
-__device__ __forceinline__ unsigned group_total(unsigned v, unsigned x, unsigned y, unsigned *buf)
-{
-  if (x == 0u) buf[y] = v;
-  __syncthreads();
-  unsigned t = 0u;
+    wave_lds_sync();
+    for (unsigned col = k + q; col < 32u; col += 16u) {
 #pragma unroll
-  for (unsigned i = 0u; i < 16u; ++i) t += buf[i];
-  __syncthreads();
-  return t;
-}
-
-// The reference's level-0 sweeps (share/device_code_nosort.hpp:4-222,
-// share/device_code_cdsort_KISS.hpp:4-260, share/device_code_cdsort.hpp:25-167)
-// with 32 lanes per pivot pair: the same operations as the Cholesky path's
-// 8-lane version (the 32-element butterfly is the same 16,8,4,2,1 order).
-template <bool SORT>
-__device__ void l0_plain(double *R, double *W, const uint8_t *S0, unsigned s0steps, unsigned nswp,
-                         int def, bool hyp, unsigned npos, unsigned x, unsigned y,
-                         unsigned *cbuf, unsigned &out_s, unsigned &out_b)
-{
-  unsigned blk_s = 0u, blk_b = 0u;
-  for (unsigned swp = 0u; swp < nswp; ++swp) {
-    unsigned my_s = 0u, my_b = 0u;
-    for (unsigned step = 0u; step < s0steps; ++step) {
-      const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
-      const double Gp = QA(R, x, p), Gq = QA(R, x, q), Vp = QA(W, x, p), Vq = QA(W, x, q);
-      double Dp = sum32(__fma_rn(Gp, Gp, 0.0));
-      double Dq = sum32(__fma_rn(Gq, Gq, 0.0));
-      const double Apq = sum32(__fma_rn(Gp, Gq, 0.0));
-      const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
-      const int transf_s = !(Apq_ < (Dpq_ * MYTOL));
-      int pdef = def;
-      if (hyp) {
-        if ((p < npos) && (q < npos)) pdef = +1;
-        else if ((p >= npos) && (q >= npos)) pdef = -1;
-        else pdef = 0;
-      }
-      int transf_b = 0;
-      if (transf_s) {
-        double c, t;
-        const bool trig = SORT ? (pdef != 0) : (!hyp || pdef);
-        transf_b = dev_rot(trig, Apq, Dp, Dq, c, t);
-        const double t_ = trig ? -t : t;
-        if (SORT && trig) {
-          Dp = __fma_rn(t, Apq, Dp);
-          Dq = __fma_rn(t_, Apq, Dq);
+      for (unsigned j = 0u; j < 8u; ++j) {
+        const unsigned x = r + 4u * j;
+        if ((x < k) || (x > col)) continue;
+        if (x < col) {
+          const double c = gv[x], s = gv[32u + x];
+          const double a = L0A(R0, x, col), b = L0A(R1, x - k, col);
+          const double ca = c * a, sb = s * b, cb = c * b, sa = s * a;
+          L0A(R0, x, col) = ca + sb;
+          L0A(R1, x - k, col) = cb - sa;
         }
-        bool keep = true;
-        if (SORT)
-          keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
-        const unsigned P = keep ? p : q, Q = keep ? q : p;
-        const double rp = __fma_rn(t_, Gq, Gp), rq = __fma_rn(t, Gp, Gq);
-        const double wp = __fma_rn(t_, Vq, Vp), wq = __fma_rn(t, Vp, Vq);
-        QA(R, x, P) = transf_b ? c * rp : rp;
-        QA(R, x, Q) = transf_b ? c * rq : rq;
-        QA(W, x, P) = transf_b ? c * wp : wp;
-        QA(W, x, Q) = transf_b ? c * wq : wq;
+        else {
+          L0A(R0, x, col) = gv[64u + x];
+          L0A(R1, x - k, col) = 0.0;
+        }
       }
-      else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
-        QA(R, x, p) = Gq;
-        QA(R, x, q) = Gp;
-        QA(W, x, p) = Vq;
-        QA(W, x, q) = Vp;
-      }
-      my_s += (unsigned)transf_s;
-      my_b += (unsigned)transf_b;
-      __syncthreads();
     }
-    const unsigned swp_s = group_total(my_s, x, y, cbuf);
-    const unsigned swp_b = group_total(my_b, x, y, cbuf + 16);
-    if (swp_s) {
-      blk_s += swp_s;
-      blk_b += swp_b;
-    }
-    else
-      break;
+    wave_lds_sync();
   }
-  out_s = blk_s;
-  out_b = blk_b;
 }
 
-// Drmac's sorted definite level-0 (posd for definite > 0, negd for < 0),
-// share/device_code_cdsort_Drmac.hpp:4-316
-__device__ void l0_drmac(double *R, double *W, const uint8_t *S0, unsigned s0steps, unsigned nswp,
-                         int definite, unsigned x, unsigned y, unsigned *cbuf, unsigned &out_s,
-                         unsigned &out_b)
+// ---------------------------------------------------------------------------
+// Drmac's sorted definite level-0 (posd for definite >= 0: the larger norm
+// first; negd: the smaller first), share/device_code_cdsort_Drmac.hpp:4-316.
+// The cosine of the pivot columns and their norms are formed overflow-safe;
+// the rotation comes from cot(2 phi) = (nq/np - np/nq) / (2 cos), and where
+// that exceeds COT_LIMIT the smaller column is made orthogonal to the larger
+// one directly.  Quad q takes pivot pair q of a step.
+__device__ void drmac_level0(double *R, double *W, const uint8_t *S0, unsigned s0steps,
+                             unsigned nswp, int definite, unsigned q, unsigned r,
+                             unsigned &out_s, unsigned &out_b)
 {
   const bool pos = (definite >= 0);
   unsigned blk_s = 0u, blk_b = 0u;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned my_s = 0u, my_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
-      const unsigned p = S0[(step * 16u + y) * 2u], q = S0[(step * 16u + y) * 2u + 1u];
-      const double Gp = QA(R, x, p), Gq = QA(R, x, q), Vp = QA(W, x, p), Vq = QA(W, x, q);
-      const double Ap = nrm2_32(Gp), Aq = nrm2_32(Gq);
-      const double CosA = cosa(Ap, Aq, Gp, Gq);
-      const int transf_s = (fabs(CosA) >= MYTOL);
-      int transf_b = 0;
-      if (transf_s) {
-        const double Aq_p = Aq / Ap, Ap_q = Ap / Aq;
-        const double ct2 = (Aq_p - Ap_q) / scalbn(CosA, 1);  // tCot2
-        if (ct2 <= QR_NU_8) {
-          const double CT2 = fabs(ct2);
+      const unsigned p = S0[(step * 16u + q) * 2u], o = S0[(step * 16u + q) * 2u + 1u];
+      double Gp[8], Go[8], Vp[8], Vo[8];
+#pragma unroll
+      for (unsigned j = 0u; j < 8u; ++j) {
+        Gp[j] = L0A(R, r + 4u * j, p);
+        Go[j] = L0A(R, r + 4u * j, o);
+        Vp[j] = L0A(W, r + 4u * j, p);
+        Vo[j] = L0A(W, r + 4u * j, o);
+      }
+      const double Ap = quad_nrm2(Gp), Ao = quad_nrm2(Go);
+      const double cs = quad_cos(Ap, Ao, Gp, Go);
+      const int rot = (fabs(cs) >= MYTOL);
+      int big = 0;
+      if (rot) {
+        const double o_p = Ao / Ap, p_o = Ap / Ao;
+        const double ct2 = (o_p - p_o) / scalbn(cs, 1);
+        if (ct2 <= COT_LIMIT) {
+          const double act = fabs(ct2);
           double c, t;
-          if (CT2 < SQRT_HEPS) {
-            const double cot = CT2 + 1.0;
-            t = copysign(1.0 / cot, ct2);
+          if (act < SQRT_HEPS) {
+            t = copysign(1.0 / (act + 1.0), ct2);
             c = dev_rsqrt(__fma_rn(t, t, 1.0));
-            transf_b = 1;
+            big = 1;
           }
-          else if (CT2 >= SQRT_2_HEPS) {
-            const double cot = scalbn(CT2, 1);
+          else if (act >= SQRT_2_HEPS) {
+            t = copysign(1.0 / scalbn(act, 1), ct2);
             c = 1.0;
-            t = copysign(1.0 / cot, ct2);
           }
           else {
-            const double cot = CT2 + sqrt(__fma_rn(CT2, CT2, 1.0));
-            t = copysign(1.0 / cot, ct2);
+            t = copysign(1.0 / (act + sqrt(__fma_rn(act, act, 1.0))), ct2);
             c = dev_rsqrt(__fma_rn(t, t, 1.0));
-            transf_b = (c < 1.0);
+            big = (c < 1.0);
           }
-          const double t_ = -t, tCosA = t * CosA, tCosA_ = -tCosA;
-          const double Ap_ = Ap * sqrt(__fma_rn(tCosA_, Aq_p, 1.0));
-          const double Aq_ = Aq * sqrt(__fma_rn(tCosA, Ap_q, 1.0));
-          const bool keep = pos ? (Ap_ >= Aq_) : (Ap_ <= Aq_);
-          const unsigned P = keep ? p : q, Q = keep ? q : p;
-          const double rp = __fma_rn(t_, Gq, Gp), rq = __fma_rn(t, Gp, Gq);
-          const double wp = __fma_rn(t_, Vq, Vp), wq = __fma_rn(t, Vp, Vq);
-          QA(R, x, P) = transf_b ? c * rp : rp;
-          QA(R, x, Q) = transf_b ? c * rq : rq;
-          QA(W, x, P) = transf_b ? c * wp : wp;
-          QA(W, x, Q) = transf_b ? c * wq : wq;
+          const double tc = t * cs;
+          // the norms after the rotation decide which column goes first
+          const double Ap2 = Ap * sqrt(__fma_rn(-tc, o_p, 1.0));
+          const double Ao2 = Ao * sqrt(__fma_rn(tc, p_o, 1.0));
+          const bool keep = pos ? (Ap2 >= Ao2) : (Ap2 <= Ao2);
+          const unsigned P1 = keep ? p : o, P2 = keep ? o : p;
+#pragma unroll
+          for (unsigned j = 0u; j < 8u; ++j) {
+            const double rp = __fma_rn(-t, Go[j], Gp[j]), ro = __fma_rn(t, Gp[j], Go[j]);
+            const double wp = __fma_rn(-t, Vo[j], Vp[j]), wo = __fma_rn(t, Vp[j], Vo[j]);
+            L0A(R, r + 4u * j, P1) = big ? c * rp : rp;
+            L0A(R, r + 4u * j, P2) = big ? c * ro : ro;
+            L0A(W, r + 4u * j, P1) = big ? c * wp : wp;
+            L0A(W, r + 4u * j, P2) = big ? c * wo : wo;
+          }
         }
-        else if (pos ? (Ap >= Aq) : (Ap <= Aq)) {
-          const double t_ = CosA, t = -t_;
-          QA(R, x, q) = Aq * __fma_rn(t, Gp / Ap, Gq / Aq);
-          QA(W, x, p) = __fma_rn(t_, Vq, Vp);
-          QA(W, x, q) = __fma_rn(t, Vp, Vq);
+        else if (pos ? (Ap >= Ao) : (Ap <= Ao)) {
+          // column o loses its component along p; p stays first
+#pragma unroll
+          for (unsigned j = 0u; j < 8u; ++j) {
+            L0A(R, r + 4u * j, o) = Ao * __fma_rn(-cs, Gp[j] / Ap, Go[j] / Ao);
+            L0A(W, r + 4u * j, p) = __fma_rn(cs, Vo[j], Vp[j]);
+            L0A(W, r + 4u * j, o) = __fma_rn(-cs, Vp[j], Vo[j]);
+          }
         }
         else {
-          const double t = CosA, t_ = -t;
-          QA(R, x, q) = Ap * __fma_rn(t_, Gq / Aq, Gp / Ap);
-          QA(R, x, p) = Gq;
-          QA(W, x, q) = __fma_rn(t_, Vq, Vp);
-          QA(W, x, p) = __fma_rn(t, Vp, Vq);
+          // column p loses its component along o, and the two change places
+#pragma unroll
+          for (unsigned j = 0u; j < 8u; ++j) {
+            L0A(R, r + 4u * j, o) = Ap * __fma_rn(-cs, Go[j] / Ao, Gp[j] / Ap);
+            L0A(R, r + 4u * j, p) = Go[j];
+            L0A(W, r + 4u * j, o) = __fma_rn(-cs, Vo[j], Vp[j]);
+            L0A(W, r + 4u * j, p) = __fma_rn(cs, Vp[j], Vo[j]);
+          }
         }
       }
-      else if (pos ? (Ap < Aq) : (Aq < Ap)) {
-        QA(R, x, p) = Gq;
-        QA(R, x, q) = Gp;
-        QA(W, x, p) = Vq;
-        QA(W, x, q) = Vp;
+      else if (pos ? (Ap < Ao) : (Ao < Ap)) {
+#pragma unroll
+        for (unsigned j = 0u; j < 8u; ++j) {
+          L0A(R, r + 4u * j, p) = Go[j];
+          L0A(R, r + 4u * j, o) = Gp[j];
+          L0A(W, r + 4u * j, p) = Vo[j];
+          L0A(W, r + 4u * j, o) = Vp[j];
+        }
       }
-      my_s += (unsigned)transf_s;
-      my_b += (unsigned)transf_b;
-      __syncthreads();
+      my_s += (unsigned)rot;
+      my_b += (unsigned)big;
+      wave_lds_sync();
     }
-    const unsigned swp_s = group_total(my_s, x, y, cbuf);
-    const unsigned swp_b = group_total(my_b, x, y, cbuf + 16);
-    if (swp_s) {
-      blk_s += swp_s;
-      blk_b += swp_b;
-    }
-    else
+    // the four lanes of a quad hold its counts
+    const unsigned swp_s = wave_sum_u(my_s) >> 2, swp_b = wave_sum_u(my_b) >> 2;
+    if (!swp_s)
       break;
+    blk_s += swp_s;
+    blk_b += swp_b;
   }
   out_s = blk_s;
   out_b = blk_b;
 }
 
-// One block pair per workgroup: the QR factorization, the level-0 sweeps,
-// the convergence counter, the update flag, W.
-template <bool SORT, bool DRMAC>
-__global__ __launch_bounds__(512) void k_pair_qr(const double *__restrict__ G, unsigned ldG,
-                                                 unsigned nrow, const uint16_t *__restrict__ pairs,
-                                                 double *__restrict__ Wout,
-                                                 unsigned *__restrict__ flags, RunCtl *__restrict__ ctl,
-                                                 const uint8_t *__restrict__ strat0, unsigned s0steps,
-                                                 unsigned nswp, int definite, unsigned nplus)
+// ---------------------------------------------------------------------------
+
+// J of the pair's 32 columns (J is +1 on the first nplus columns of G):
+// npos of them are positive, and they come first; definite is +1 / -1 when
+// all / none are, else 0 (share/device_code_nosort_accumV.hpp:53-76, whose
+// case nplus = 16 bj + 16 stays mixed with npos = 32).
+__device__ __forceinline__ void pair_signs(int definite, unsigned nplus, unsigned bi, unsigned bj,
+                                           int &def, bool &hyp, unsigned &npos)
 {
-  __shared__ double A[32u * QLD];
-  __shared__ double P[32u * QLD];
+  def = definite;
+  hyp = false;
+  npos = 0u;
+  if (definite)
+    return;
+  if (nplus <= 16u * bi) {
+    def = -1;
+  }
+  else if (nplus > 16u * bj + 16u) {
+    def = +1;
+    npos = 32u;
+  }
+  else {
+    def = 0;
+    hyp = true;
+    npos = min(nplus - 16u * bi, 16u) + ((nplus > 16u * bj) ? nplus - 16u * bj : 0u);
+  }
+}
+
+// One block pair per 64-lane workgroup.
+template <bool SORT, bool DRMAC>
+__global__ __launch_bounds__(64) void k_pair_qr(const double *__restrict__ G, unsigned ldG,
+                                                unsigned nrow, const uint16_t *__restrict__ pairs,
+                                                double *__restrict__ Wout,
+                                                unsigned *__restrict__ flags, RunCtl *__restrict__ ctl,
+                                                const uint8_t *__restrict__ strat0, unsigned s0steps,
+                                                unsigned nswp, int definite, unsigned nplus)
+{
+  __shared__ double R[32u * L0LD];
+  __shared__ double P[32u * L0LD];
+  __shared__ double gv[96];
   __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
-  __shared__ unsigned cbuf[32];
-  const unsigned tid = threadIdx.x, x = tid & 31u, y = tid >> 5;
+  const unsigned l = threadIdx.x, q = l >> 2, r = l & 3u;
   const unsigned pi = blockIdx.x;
-  const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
-  if (tid < STRAT0_TAB_BYTES / 16u)
-    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(strat0)[tid];
+  const unsigned bi = pairs[2u * pi], bj = pairs[2u * pi + 1u];
+  static_assert(STRAT0_TAB_BYTES == 64u * sizeof(uint4), "one 16-byte piece per lane");
+  reinterpret_cast<uint4 *>(S0)[l] = reinterpret_cast<const uint4 *>(strat0)[l];
 
-  // local J structure, share/device_code_nosort_accumV.hpp:53-76
-  bool hyp = false;
-  int def = definite;
-  unsigned npos = 0u;
-  if (!definite) {
-    const unsigned ifc0 = iBlk * 16u, ifc1 = jBlk * 16u;
-    if (nplus <= ifc0) { npos = 0u; def = -1; }
-    else if (nplus <= ifc0 + 16u) { npos = nplus - ifc0; def = 0; }
-    else if (nplus <= ifc1) { npos = 16u; def = 0; }
-    else if (nplus <= ifc1 + 16u) { npos = 16u + (nplus - ifc1); def = 0; }
-    else { npos = 32u; def = +1; }
-    hyp = (def == 0);
-  }
+  int def;
+  bool hyp;
+  unsigned npos;
+  pair_signs(definite, nplus, bi, bj, def, hyp, npos);
 
-  const double *const G0 = G + (size_t)(16u * iBlk + y) * ldG;
-  const double *const G1 = G + (size_t)(16u * jBlk + y) * ldG;
-  QA(A, x, y) = G0[x];
-  QA(A, x, y + 16u) = G1[x];
-  __syncthreads();
-  qr32(A, x, y);
+  // panel loads: lane l takes row l % 32 of the pair's columns 2i + l / 32
+  // (i < 8 in block bi, i >= 8 in block bj)
+  const unsigned row = l & 31u, half = l >> 5;
+  const double *const c0 = G + (size_t)(16u * bi + half) * ldG + row;
+  const double *const c1 = G + (size_t)(16u * bj + half) * ldG + row;
+  auto load_panel = [&](double (&v)[16], unsigned i0) {
+#pragma unroll
+    for (unsigned i = 0u; i < 16u; ++i)
+      v[i] = ((i < 8u) ? c0 : c1)[(size_t)(2u * (i & 7u)) * ldG + i0];
+  };
+  double pf[16];
+  load_panel(pf, 0u);
+#pragma unroll
+  for (unsigned i = 0u; i < 16u; ++i) L0A(R, row, 2u * i + half) = pf[i];
+  if (nrow > 32u)
+    load_panel(pf, 32u);
+  wave_lds_sync();
+  householder32(R, q, r);
 #pragma unroll 1
-  for (unsigned i = 32u; i < nrow; i += 32u) {
-    QA(P, x, y) = G0[i + x];
-    QA(P, x, y + 16u) = G1[i + x];
-    __syncthreads();
-    qr32(P, x, y);
-    peeloff(A, P, x, y);
+  for (unsigned i0 = 32u; i0 < nrow; i0 += 32u) {
+#pragma unroll
+    for (unsigned i = 0u; i < 16u; ++i) L0A(P, row, 2u * i + half) = pf[i];
+    if (i0 + 32u < nrow)
+      load_panel(pf, i0 + 32u);
+    wave_lds_sync();
+    householder32(P, q, r);
+    merge_rows(R, P, gv, l, q, r);
   }
-  QA(P, x, y) = (x == y) ? 1.0 : 0.0;
-  QA(P, x, y + 16u) = (x == y + 16u) ? 1.0 : 0.0;
-  __syncthreads();
+  // W = I
+#pragma unroll
+  for (unsigned i = 0u; i < 16u; ++i) {
+    const unsigned c = 2u * i + half;
+    L0A(P, row, c) = (row == c) ? 1.0 : 0.0;
+  }
+  wave_lds_sync();
 
   unsigned blk_s = 0u, blk_b = 0u;
-  if (DRMAC && SORT && !hyp)
-    l0_drmac(A, P, S0, s0steps, nswp, def, x, y, cbuf, blk_s, blk_b);
-  else
-    l0_plain<SORT>(A, P, S0, s0steps, nswp, def, hyp, npos, x, y, cbuf, blk_s, blk_b);
+  if (DRMAC && SORT && !hyp) {
+    drmac_level0(R, P, S0, s0steps, nswp, def, q, r, blk_s, blk_b);
+  }
+  else {
+    double unused_cos;
+    wave_level0<SORT, false>(R, P, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, unused_cos);
+  }
+  wave_lds_sync();
 
-  if (tid == 0u) {
-    // convergence counter, share/device_code_nosort.hpp:94-103
-    unsigned long long *const cvg = &ctl->cvg;
-    if (blk_s) {
-      if (blk_b)
-        atomicAdd(cvg, (((unsigned long long)blk_b) << 32) | blk_s);
-      else
-        atomicAdd(reinterpret_cast<unsigned *>(cvg), blk_s);
-    }
+  if (l == 0u) {
+    // the run's counter: big rotations in the high word, all in the low one;
+    // without a big rotation only the low word is added to, with no carry
+    // out of it (share/device_code_nosort.hpp:94-103)
+    if (blk_s && blk_b)
+      atomicAdd(&ctl->cvg, (((unsigned long long)blk_b) << 32) | blk_s);
+    else if (blk_s)
+      atomicAdd(reinterpret_cast<unsigned *>(&ctl->cvg), blk_s);
     flags[pi] = SORT ? 1u : (blk_s ? 1u : 0u);
   }
   double *const wo = Wout + (size_t)pi * 1024u;
-  wo[y * 32u + x] = QA(P, x, y);
-  wo[(y + 16u) * 32u + x] = QA(P, x, y + 16u);
+#pragma unroll
+  for (unsigned i = 0u; i < 16u; ++i) {
+    const unsigned c = 2u * i + half;
+    wo[c * 32u + row] = L0A(P, row, c);
+  }
 }
 
 }  // namespace
@@ -589,13 +570,13 @@ void launch_pair_qr(hipStream_t s, bool sort, bool drmac, const double *G, unsig
                     int definite, unsigned nplus)
 {
   if (sort && drmac)
-    hipLaunchKernelGGL((k_pair_qr<true, true>), dim3(np), dim3(512), 0, s, G, ldG, nrow, pairs, W,
+    hipLaunchKernelGGL((k_pair_qr<true, true>), dim3(np), dim3(64), 0, s, G, ldG, nrow, pairs, W,
                        flags, ctl, strat0, s0steps, nswp, definite, nplus);
   else if (sort)
-    hipLaunchKernelGGL((k_pair_qr<true, false>), dim3(np), dim3(512), 0, s, G, ldG, nrow, pairs, W,
+    hipLaunchKernelGGL((k_pair_qr<true, false>), dim3(np), dim3(64), 0, s, G, ldG, nrow, pairs, W,
                        flags, ctl, strat0, s0steps, nswp, definite, nplus);
   else
-    hipLaunchKernelGGL((k_pair_qr<false, false>), dim3(np), dim3(512), 0, s, G, ldG, nrow, pairs, W,
+    hipLaunchKernelGGL((k_pair_qr<false, false>), dim3(np), dim3(64), 0, s, G, ldG, nrow, pairs, W,
                        flags, ctl, strat0, s0steps, nswp, definite, nplus);
 }
 

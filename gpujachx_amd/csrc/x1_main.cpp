@@ -1,146 +1,211 @@
-// x1_main.cpp -- command-line test driver with the reference's surface
-// (1/main.cpp is 1/main.cu:10-160):
+// x1_main.cpp -- the x1 driver: one (hyperbolic) SVD of an HDF5 input on one GPU.
 //
 //   x1 DEV SDY SNP0 SNP1 ALG H5F H5G [H5R]
 //
-// reads /H5G/IDADIM = {LDA, M, N, NPLUS} (int32[4]) and /H5G/G (float64,
-// Fortran LDA x N) from H5F, runs hypJacL2 on device DEV with the strategy
-// tables SNP0 (order 32) and SNP1 (order ceil(H5G/16)) from the plugin SDY,
-// prints the per-sweep lines and the totals, and, if H5R is given, writes a
-// new group H5G there with IDADIM, ISTATS, TIMING, G, V (if ALG & 2) and D.
-// The reference's build options USE_QR / USE_DRMAC are chosen at run time by
-// the environment variable HJ_FACT = qr | drmac (default: Cholesky).
+// What it keeps of the reference's driver is its surface, not its code:
+//  * the usage line and the argument meaning (1/main.cu:12-15): device,
+//    strategy plugin, level-0 and level-1 family names, ALG, input file,
+//    group, optional result file; the level-1 order is ceil(H5G / 16), so
+//    the group must be named by the column count (1/main.cu:32);
+//  * the input group (1/main.cu:54-60): IDADIM int32[4] = {LDA, M, N, NPLUS}
+//    and G float64 with C dimensions [N][LDA] (Fortran LDA x N);
+//  * stdout: the library's per-sweep BLK_SWP lines, then the GLB_ROT_S /
+//    GLB_ROT_B totals and the wall time with the sweep count (1/main.cu:98-100);
+//  * the result group, created anew in H5R (1/main.cu:105-140): IDADIM,
+//    ISTATS int32[4] = {sweeps or the error code, STRAT0, STRAT1, return
+//    code}, TIMING float64[4], G [N][LDA], V [N][LDA] when ALG & 2, D [N];
+//  * the exit status is hypJacL2's return code.
+// The reference's compile-time factorization variants (USE_QR, USE_DRMAC)
+// are chosen at run time by HJ_FACT = cholesky | qr | drmac.
 #include <hdf5.h>
 #include <hdf5_hl.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "hypjac.h"
 
-#define H5CALL(call)                                                                 \
-  do {                                                                               \
-    const herr_t e_ = static_cast<herr_t>(call);                                     \
-    if (e_ < 0) {                                                                    \
-      (void)fprintf(stderr, "HDF5 error %d @ %s(%d)!\n", (int)e_, __FILE__, __LINE__); \
-      exit(EXIT_FAILURE);                                                            \
-    }                                                                                \
-  } while (0)
+namespace {
 
-#define X1_DIE(msg)                                                              \
-  do {                                                                           \
-    (void)fprintf(stderr, "[ERROR] %s(%d): %s\n", __FILE__, __LINE__, (msg));    \
-    exit(EXIT_FAILURE);                                                          \
-  } while (0)
+[[noreturn]] void fail(const std::string &what)
+{
+  (void)fprintf(stderr, "x1: %s\n", what.c_str());
+  exit(EXIT_FAILURE);
+}
+
+// An HDF5 status or identifier; negative values are failures.
+template <typename T>
+T h5(T v, const char *what)
+{
+  if (v < 0)
+    fail(std::string("HDF5 failed: ") + what);
+  return v;
+}
+
+// Owns one open HDF5 file or group.
+class H5Object {
+ public:
+  H5Object(hid_t id, herr_t (*closer)(hid_t)) : id_(id), closer_(closer) {}
+  H5Object(const H5Object &) = delete;
+  H5Object &operator=(const H5Object &) = delete;
+  ~H5Object()
+  {
+    if (id_ >= 0) (void)closer_(id_);
+  }
+  hid_t id() const { return id_; }
+
+ private:
+  hid_t id_;
+  herr_t (*closer_)(hid_t);
+};
+
+struct Options {
+  int device = 0;
+  const char *plugin = nullptr;  // strat.so
+  const char *fam0 = nullptr;    // level-0 family (order 32)
+  const char *fam1 = nullptr;    // level-1 family (order ceil(N / 16))
+  unsigned alg = 0u;
+  const char *input = nullptr;
+  const char *group = nullptr;
+  const char *result = nullptr;  // optional
+};
+
+bool parse(int argc, char *argv[], Options &o)
+{
+  if (argc != 8 && argc != 9)
+    return false;
+  o.device = atoi(argv[1]);
+  o.plugin = argv[2];
+  o.fam0 = argv[3];
+  o.fam1 = argv[4];
+  o.alg = static_cast<unsigned>(atoi(argv[5]));
+  o.input = argv[6];
+  o.group = argv[7];
+  o.result = (argc == 9) ? argv[8] : nullptr;
+  return true;
+}
+
+void choose_factorization()
+{
+  const char *f = getenv("HJ_FACT");
+  if (!f || !strcmp(f, "cholesky"))
+    return;
+  if (!strcmp(f, "qr"))
+    (void)hj_set_factorization(HJ_FACT_QR);
+  else if (!strcmp(f, "drmac"))
+    (void)hj_set_factorization(HJ_FACT_QR_DRMAC);
+  else
+    fail("HJ_FACT must be cholesky, qr or drmac");
+}
+
+// The input problem: G with leading dimension lda (>= m), n columns, the
+// first nplus of them with J = +1.
+struct Problem {
+  unsigned lda = 0u, m = 0u, n = 0u, nplus = 0u;
+  std::vector<double> G;
+};
+
+Problem load(const char *path, const char *group)
+{
+  if (!fexist(path))
+    fail(std::string("no input file ") + path);
+  H5Object file(h5(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), "open input"), H5Fclose);
+  H5Object grp(h5(H5Gopen2(file.id(), group, H5P_DEFAULT), "open input group"), H5Gclose);
+  int dim[HYPJAC_IDADIM_SIZE] = {0, 0, 0, 0};
+  h5(H5LTread_dataset_int(grp.id(), "IDADIM", dim), "read IDADIM");
+  for (int d : dim)
+    if (d < 0)
+      fail("IDADIM holds a negative size");
+  Problem p;
+  p.lda = static_cast<unsigned>(dim[0]);
+  p.m = static_cast<unsigned>(dim[1]);
+  p.n = static_cast<unsigned>(dim[2]);
+  p.nplus = static_cast<unsigned>(dim[3]);
+  if (p.lda < p.m)
+    fail("IDADIM: LDA is below M");
+  p.G.assign(static_cast<size_t>(p.lda) * p.n, 0.0);
+  h5(H5LTread_dataset_double(grp.id(), "G", p.G.data()), "read G");
+  return p;
+}
+
+struct Result {
+  int ret = 0;
+  unsigned sweeps = 0u;
+  unsigned long long rot_s = 0ull, rot_b = 0ull;
+  double timing[4] = {-0.0, -0.0, -0.0, -0.0};
+  std::vector<double> V, D;
+};
+
+// Runs hypJacL2 in place on p.G (which becomes U); V gets the input's
+// leading dimension, as the reference's driver allocates it.
+Result solve(unsigned alg, Problem &p)
+{
+  Result r;
+  const bool withV = (alg & HYPJAC_FULL_SVD) != 0u;
+  if (withV)
+    r.V.assign(static_cast<size_t>(p.lda) * p.n, 0.0);
+  r.D.assign(p.n, 0.0);
+  r.ret = hypJacL2(alg, p.m, p.n, p.nplus, p.G.data(), p.lda, withV ? r.V.data() : nullptr,
+                   withV ? p.lda : 0u, r.D.data(), &r.sweeps, &r.rot_s, &r.rot_b, r.timing);
+  return r;
+}
+
+void store(const char *path, const char *group, unsigned alg, const Problem &p, const Result &r)
+{
+  H5Object file(h5(fexist(path) ? H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT)
+                                : H5Fcreate(path, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT),
+                   "open result file"),
+                H5Fclose);
+  H5Object grp(h5(H5Gcreate2(file.id(), group, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                  "create result group"),
+               H5Gclose);
+  const int dim[HYPJAC_IDADIM_SIZE] = {static_cast<int>(p.lda), static_cast<int>(p.m),
+                                       static_cast<int>(p.n), static_cast<int>(p.nplus)};
+  const int stats[4] = {r.ret < 0 ? r.ret : static_cast<int>(r.sweeps), static_cast<int>(STRAT0),
+                        static_cast<int>(STRAT1), r.ret};
+  const hsize_t four[1] = {4u};
+  h5(H5LTmake_dataset_int(grp.id(), "IDADIM", 1, four, dim), "write IDADIM");
+  h5(H5LTmake_dataset_int(grp.id(), "ISTATS", 1, four, stats), "write ISTATS");
+  h5(H5LTmake_dataset_double(grp.id(), "TIMING", 1, four, r.timing), "write TIMING");
+  const hsize_t mat[2] = {p.n, p.lda};
+  h5(H5LTmake_dataset_double(grp.id(), "G", 2, mat, p.G.data()), "write G");
+  if (alg & HYPJAC_FULL_SVD)
+    h5(H5LTmake_dataset_double(grp.id(), "V", 2, mat, r.V.data()), "write V");
+  const hsize_t vec[1] = {p.n};
+  h5(H5LTmake_dataset_double(grp.id(), "D", 1, vec, r.D.data()), "write D");
+}
+
+}  // namespace
 
 int main(int argc, char *argv[])
 {
-  if ((8 > argc) || (9 < argc)) {
+  Options o;
+  if (!parse(argc, argv, o)) {
     (void)fprintf(stderr, "%s DEV SDY SNP0 SNP1 ALG H5F H5G [H5R]\n", argv[0]);
     return EXIT_FAILURE;
   }
-  const char *const ca_exe = argv[0];
-  const char *const ca_dev = argv[1];
-  const char *const ca_sdy = argv[2];
-  const char *const ca_snp0 = argv[3];
-  const char *const ca_snp1 = argv[4];
-  const char *const ca_alg = argv[5];
-  const char *const ca_h5f = argv[6];
-  const char *const ca_h5g = argv[7];
-  const char *const ca_h5r = (9 == argc) ? argv[8] : nullptr;
+  choose_factorization();
+  const unsigned order0 = 2u * HYPJACL1_NCOLB;
+  const unsigned order1 = (static_cast<unsigned>(atoi(o.group)) + HYPJACL1_NCOLB - 1u) / HYPJACL1_NCOLB;
+  init_strats(o.plugin, o.fam0, order0, o.fam1, order1);
+  (void)configureGPU(o.device);  // ends the process on a device it cannot use
 
-  const unsigned n0 = HYPJACL1_NCOLB << 1u;
-  const unsigned n1 = (static_cast<unsigned>(atoi(ca_h5g)) + HYPJACL1_NCOLB - 1u) / HYPJACL1_NCOLB;
-  init_strats(ca_sdy, ca_snp0, n0, ca_snp1, n1);
-
-  if (const char *f = getenv("HJ_FACT")) {
-    if (!strcmp(f, "qr"))
-      (void)hj_set_factorization(HJ_FACT_QR);
-    else if (!strcmp(f, "drmac"))
-      (void)hj_set_factorization(HJ_FACT_QR_DRMAC);
-    else if (strcmp(f, "cholesky"))
-      X1_DIE("HJ_FACT must be qr, drmac or cholesky");
+  h5(H5open(), "initialise");
+  Problem p = load(o.input, o.group);
+  const Result r = solve(o.alg, p);
+  if (r.ret) {
+    (void)fprintf(stderr, "%s: error %d\n", argv[0], r.ret);
   }
-
-  const int dev = atoi(ca_dev);
-  const int dcc = configureGPU(dev);
-  if (dcc < 30) {
-    char msg[128];
-    (void)snprintf(msg, sizeof(msg), "Device %d has CC %d < 30", dev, dcc);
-    X1_DIE(msg);
-  }
-  const unsigned routine = static_cast<unsigned>(atoi(ca_alg));
-  if (!fexist(ca_h5f)) {
-    char msg[1024];
-    (void)snprintf(msg, sizeof(msg), "non-existent H5F(%s)", ca_h5f);
-    X1_DIE(msg);
-  }
-
-  H5CALL(H5open());
-  hid_t fid = -1, gid = -1;
-  H5CALL(fid = H5Fopen(ca_h5f, H5F_ACC_RDONLY, H5P_DEFAULT));
-  H5CALL(gid = H5Gopen2(fid, ca_h5g, H5P_DEFAULT));
-  int idadim[HYPJAC_IDADIM_SIZE] = {0, 0, 0, 0};
-  H5CALL(H5LTread_dataset_int(gid, "IDADIM", idadim));
-  unsigned ldhG = static_cast<unsigned>(idadim[0]);
-  const unsigned nrow = static_cast<unsigned>(idadim[1]);
-  const unsigned ncol = static_cast<unsigned>(idadim[2]);
-  const unsigned nplus = static_cast<unsigned>(idadim[3]);
-  if (ldhG < nrow) X1_DIE("IDADIM: LDA < M");
-  std::vector<double> hG((size_t)ldhG * ncol, 0.0);
-  H5CALL(H5LTread_dataset_double(gid, "G", hG.data()));
-  H5CALL(H5Gclose(gid));
-  H5CALL(H5Fclose(fid));
-
-  const bool full = (routine & HYPJAC_FULL_SVD) != 0u;
-  std::vector<double> hV;
-  unsigned ldhV = 0u;
-  if (full) {
-    hV.assign((size_t)ldhG * ncol, 0.0);
-    ldhV = ldhG;
-  }
-  std::vector<double> hD(ncol, 0.0);
-  unsigned glbSwp = 0u;
-  unsigned long long glb_s = 0ull, glb_b = 0ull;
-  double timing[4] = {-0.0, -0.0, -0.0, -0.0};
-  const int ret = hypJacL2(routine, nrow, ncol, nplus, hG.data(), ldhG, full ? hV.data() : nullptr,
-                           ldhV, hD.data(), &glbSwp, &glb_s, &glb_b, timing);
-  if (ret)
-    (void)fprintf(stderr, "%s: error %d\n", ca_exe, ret);
   else {
-    (void)fprintf(stdout, "GLB_ROT_S(%20llu), GLB_ROT_B(%20llu)\n", glb_s, glb_b);
-    (void)fflush(stdout);
-    (void)fprintf(stdout, "%#12.6f s %2u sweeps\n", *timing, glbSwp);
+    (void)printf("GLB_ROT_S(%20llu), GLB_ROT_B(%20llu)\n", r.rot_s, r.rot_b);
+    (void)printf("%#12.6f s %2u sweeps\n", r.timing[0], r.sweeps);
     (void)fflush(stdout);
   }
-
-  if (ca_h5r) {
-    H5CALL(fid = fexist(ca_h5r) ? H5Fopen(ca_h5r, H5F_ACC_RDWR, H5P_DEFAULT)
-                                : H5Fcreate(ca_h5r, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT));
-    H5CALL(gid = H5Gcreate2(fid, ca_h5g, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT));
-    idadim[0] = static_cast<int>(ldhG);
-    hsize_t dims[2] = {HYPJAC_IDADIM_SIZE, 1};
-    H5CALL(H5LTmake_dataset_int(gid, "IDADIM", 1, dims, idadim));
-    int istats[4] = {(ret < 0) ? ret : static_cast<int>(glbSwp), static_cast<int>(STRAT0),
-                     static_cast<int>(STRAT1), ret};
-    dims[0] = 4;
-    H5CALL(H5LTmake_dataset_int(gid, "ISTATS", 1, dims, istats));
-    H5CALL(H5LTmake_dataset_double(gid, "TIMING", 1, dims, timing));
-    dims[0] = ncol;
-    dims[1] = ldhG;
-    H5CALL(H5LTmake_dataset_double(gid, "G", 2, dims, hG.data()));
-    if (full) {
-      dims[1] = ldhV;
-      H5CALL(H5LTmake_dataset_double(gid, "V", 2, dims, hV.data()));
-    }
-    dims[0] = ncol;
-    dims[1] = 1;
-    H5CALL(H5LTmake_dataset_double(gid, "D", 1, dims, hD.data()));
-    H5CALL(H5Gclose(gid));
-    H5CALL(H5Fclose(fid));
-  }
-  H5CALL(H5close());
-  return ret;
+  if (o.result)
+    store(o.result, o.group, o.alg, p, r);
+  h5(H5close(), "close");
+  return r.ret;
 }
