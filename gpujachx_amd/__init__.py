@@ -70,6 +70,8 @@ def lib():
     L.hypJacL2.restype = i
     L.hypJacL2_dev.argtypes = [u, u, u, u, vp, u, vp, u, vp, pu, pull, pull, pd, vp]
     L.hypJacL2_dev.restype = i
+    L.hypJacL2_dev_noalloc.argtypes = [u, u, u, u, vp, u, vp, u, vp, pu, pull, pull, pd, vp]
+    L.hypJacL2_dev_noalloc.restype = i
     L.init_strats.argtypes = [cp, cp, u, cp, u]
     L.init_strats.restype = None
     L.hj_init_strats_builtin.argtypes = [cp, u, cp, u]

@@ -1084,6 +1084,189 @@ extern "C" int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol, unsi
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free level-2 run: every work buffer is carved from `scratch`,
+// a fixed device array of the library (each GPU holds its own copy until the
+// process ends).  In host code the array's name only stands for the symbol;
+// its device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and only that address is handed to kernels and
+// copies.  Calls that overlap on one GPU share the array and race.
+
+constexpr size_t NOALLOC_BYTES = (size_t)8u << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+namespace {
+
+__global__ void k_zero_words(unsigned *p, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// the scratch array's device address on the current device
+unsigned char *scratch_here()
+{
+  int dev = 0;
+  HIP_CALL(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    HJ_DIE("hypJacL2_dev_noalloc: device number beyond the scratch table");
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_CALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<unsigned char *>(g_scratch_ptr[dev]);
+}
+
+size_t align256(size_t b) { return (b + 255u) & ~(size_t)255u; }
+
+// Byte offsets of the work buffers of a step-by-step level-2 run in the
+// scratch array; total() is what the run needs.
+struct ScratchPlan {
+  size_t ctl, s0, s1, flags, W, part, rec, end;
+  ScratchPlan(unsigned np, unsigned steps, unsigned nchunk)
+  {
+    ctl = 0u;
+    s0 = align256(ctl + sizeof(RunCtl));
+    s1 = align256(s0 + STRAT0_TAB_BYTES);
+    flags = align256(s1 + sizeof(uint16_t) * 2u * np * steps);
+    W = align256(flags + sizeof(unsigned) * np);
+    part = align256(W + sizeof(double) * 1024u * np);
+    rec = align256(part + sizeof(double) * 768u * nchunk * np);
+    end = align256(rec + sizeof(VisitRec) * (size_t)np * steps);
+  }
+};
+
+}  // namespace
+
+// The sweeps of level2_core in the plain FP64 form, each level-1 step
+// launched as Gram, pair visits and block update on the caller's stream,
+// with the control block read back after every sweep.
+static void level2_noalloc(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
+                           double *dG, unsigned lddG, double *dV, unsigned lddV, double *dD,
+                           unsigned *glbSwp, unsigned long long *glb_s,
+                           unsigned long long *glb_b, hipStream_t s, unsigned char *base,
+                           const ScratchPlan &sp)
+{
+  const bool cdsort = (routine & HYPJAC_CDSORT_1) != 0u;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  const unsigned nswp = (routine & HYPJAC_BLK_ORI) ? 1u : HYPJAC_NSWEEP;
+  Work w;  // views into the scratch array; nothing is allocated or freed
+  w.np = ncol / 32u;
+  w.steps = STRAT1_STEPS;
+  w.s0steps = STRAT0_STEPS;
+  w.nchunk = gram_nchunk(nrow, w.np);
+  w.CH = nrow / w.nchunk;
+  w.chG = update_rows(nrow, full_svd ? ncol : 0u, w.np);
+  w.chV = w.chG;
+  w.d_ctl = reinterpret_cast<RunCtl *>(base + sp.ctl);
+  w.d_s0 = base + sp.s0;
+  w.d_s1 = reinterpret_cast<uint16_t *>(base + sp.s1);
+  w.d_flags = reinterpret_cast<unsigned *>(base + sp.flags);
+  w.d_W = reinterpret_cast<double *>(base + sp.W);
+  w.d_part = reinterpret_cast<double *>(base + sp.part);
+  w.d_rec = reinterpret_cast<VisitRec *>(base + sp.rec);
+  // tables straight from the host arrays (strat1 rows hold STRAT1_MAX_PAIRS
+  // pairs; the run's np of each row are taken)
+  HIP_CALL(hipMemcpyAsync(w.d_s0, &strat0[0][0][0], STRAT0_TAB_BYTES, hipMemcpyHostToDevice, s));
+  HIP_CALL(hipMemcpy2DAsync(w.d_s1, sizeof(uint16_t) * 2u * w.np, &strat1[0][0][0],
+                            sizeof(uint16_t) * 2u * STRAT1_MAX_PAIRS, sizeof(uint16_t) * 2u * w.np,
+                            w.steps, hipMemcpyHostToDevice, s));
+  // the control block and the visit records start at zero (the array keeps
+  // the last call's contents)
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(w.d_ctl),
+                     sizeof(RunCtl) / sizeof(unsigned));
+  const size_t nrec = sizeof(VisitRec) * w.keys() / sizeof(unsigned);
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((nrec + 255u) / 256u, 1024u)),
+                     dim3(256), 0, s, reinterpret_cast<unsigned *>(w.d_rec), nrec);
+  HIP_CALL(hipGetLastError());
+  if (full_svd)
+    launch_initV(s, dV, 0u, ncol, lddV);
+  const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
+  double *const V = full_svd ? dV : nullptr;
+
+  g_stats = Stats();
+  *glb_s = 0ull;
+  *glb_b = 0ull;
+  long long swp_tim = 0;
+  stopwatch_reset(swp_tim);
+  unsigned blk_swp = 0u;
+  RunCtl h;
+  while (blk_swp < g_max_sweeps) {
+    launch_sweep_begin(s, w.d_ctl);
+    for (unsigned st = 0u; st < w.steps; ++st)
+      issue_step(s, w, st, cdsort, nswp, definite, nplus, dG, lddG, nrow, V, lddV, ncol, nullptr,
+                 false, false);
+    HIP_CALL(hipMemcpyAsync(&h, w.d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    ++blk_swp;
+    const unsigned cvg_s = static_cast<unsigned>(h.cvg);
+    const unsigned cvg_b = static_cast<unsigned>(h.cvg >> 32u);
+    *glb_s += cvg_s;
+    *glb_b += cvg_b;
+    const double tim_s = stopwatch_lap(swp_tim) * TS2S;
+    if (blk_swp <= HJ_MAX_SWEEP_CAP) {
+      g_stats.rot_s[blk_swp - 1u] = cvg_s;
+      g_stats.rot_b[blk_swp - 1u] = cvg_b;
+      g_stats.sec[blk_swp - 1u] = tim_s;
+    }
+    g_stats.nswp = blk_swp;
+    g_stats.visits += w.keys();
+    if (!quiet()) {
+      (void)fprintf(stdout, "BLK_SWP(%2u), ROT_S(%10u), ROT_B(%10u), TIME(%#12.6f s)\n", blk_swp,
+                    cvg_s, cvg_b, tim_s);
+      (void)fflush(stdout);
+    }
+    if (!cvg_b)
+      break;
+  }
+  *glbSwp = blk_swp;
+  launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
+  HIP_CALL(hipStreamSynchronize(s));
+  g_last_flow = 0;
+  g_last_stage = 0;
+}
+
+extern "C" int hypJacL2_dev_noalloc(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
+                                    double *dG, unsigned lddG, double *dV, unsigned lddV,
+                                    double *dD, unsigned *glbSwp, unsigned long long *glb_s,
+                                    unsigned long long *glb_b, double *timing, void *stream)
+{
+  long long t0 = 0, t1 = 0;
+  stopwatch_reset(t0);
+  const int rc = check_args(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b);
+  if (rc)
+    return rc;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  if ((lddG & 1u) || (reinterpret_cast<uintptr_t>(dG) & 15u)) return -13;
+  if (full_svd && ((lddV & 1u) || (reinterpret_cast<uintptr_t>(dV) & 15u))) return -13;
+  check_tables(ncol);
+  const unsigned np = ncol / 32u;
+  const ScratchPlan sp(np, STRAT1_STEPS, gram_nchunk(nrow, np));
+  if (sp.end > NOALLOC_BYTES) {
+    if (!quiet())
+      (void)fprintf(stderr, "hypJacL2_dev_noalloc: ncol %u needs %zu bytes of work buffers, "
+                            "more than the %zu of the scratch array\n", ncol, sp.end, NOALLOC_BYTES);
+    return HJ_ERR_SCRATCH;
+  }
+  unsigned char *const base = scratch_here();
+  stopwatch_reset(t1);
+  level2_noalloc(routine, nrow, ncol, nplus, dG, lddG, full_svd ? dV : nullptr, lddV, dD, glbSwp,
+                 glb_s, glb_b, static_cast<hipStream_t>(stream), base, sp);
+  const long long tc = stopwatch_lap(t1);
+  const long long tw = stopwatch_lap(t0);
+  if (timing) {
+    timing[0] = tw * TS2S;
+    timing[1] = 0.0;
+    timing[2] = tc * TS2S;
+    timing[3] = 0.0;
+  }
+  return 0;
+}
+
 extern "C" void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b, double *sec)
 {
   if (nswp) *nswp = g_stats.nswp;
@@ -1287,6 +1470,7 @@ extern "C" int configureGPUex(const int dev, const unsigned maxShMemB)
     (void)snprintf(msg, sizeof(msg), "HIP device %d is %s, not a gfx9 (CDNA) part", dev, p.gcnArchName);
     HJ_DIE(msg);
   }
+  (void)scratch_here();  // the allocation-free entry's array on this device
   return p.major * 10 + p.minor;
 }
 

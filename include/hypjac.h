@@ -120,6 +120,25 @@ int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol,
                  unsigned long long *glb_s, unsigned long long *glb_b,
                  double *timing, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc, code that must not
+ * reach the allocator): hypJacL2_dev's computation with every work buffer
+ * taken from a fixed 8 MiB device array of the library (one per GPU, held
+ * until the process ends; its address is looked up by configureGPU or at the
+ * first call on a device), launched step by step on `stream` in the plain
+ * FP64 form (the forms of hj_set_form are not used).  Allocates no device or
+ * pinned memory, creates no stream, event or graph.  Results equal
+ * hypJacL2_dev's plain form bit for bit.  Same return codes as
+ * hypJacL2_dev, and HJ_ERR_SCRATCH when the run's work buffers exceed the
+ * array (ncol <= 2048 always fits, at any nrow).  Calls that overlap on one
+ * GPU share the array and race; calls in sequence are safe. */
+#define HJ_ERR_SCRATCH (-14)
+int hypJacL2_dev_noalloc(unsigned routine, unsigned nrow, unsigned ncol,
+                         unsigned nplus, double *dG, unsigned lddG, double *dV,
+                         unsigned lddV, double *dD, unsigned *glbSwp,
+                         unsigned long long *glb_s, unsigned long long *glb_b,
+                         double *timing, void *stream);
+
 /* Forms of the method (hj_set_form, bit set; 0 = the reference's method). */
 #define HJ_FORM_LEARNED_SKIP 1u  /* skip visits the pair-skip model predicts
                                     rotation-free; unskipped certification */
