@@ -160,7 +160,7 @@ __device__ void wave_update(double *__restrict__ M, unsigned ld, unsigned nrow, 
 // lower triangle): Cholesky, level-0 sweep(s), W in W.  Returns the rotation
 // counts of the pair (s, b) in every lane.
 template <bool SORT>
-__device__ void wave_visit(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
+__device__ __noinline__ void wave_visit(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
                            const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
                            unsigned l, unsigned &out_s, unsigned &out_b)
 {
@@ -179,20 +179,7 @@ __device__ void wave_visit(const BProb &P, unsigned iBlk, unsigned jBlk, double 
   }
 
   // Cholesky, share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT)
-  {
-    const unsigned x = l & 31u;
-    double a[32];
-#pragma unroll
-    for (unsigned j = 0u; j < 32u; ++j)
-      a[j] = (j <= x) ? BR(R, x, j) : 0.0;
-    chol32_rows(a, x, W);  // W is free until the identity is written below
-    __builtin_amdgcn_wave_barrier();
-    if (l < 32u) {
-#pragma unroll
-      for (unsigned y = 0u; y < 32u; ++y)
-        BR(R, y, x) = (y <= x) ? a[y] : 0.0;
-    }
-  }
+  (void)chol32_wave<BLDR>(R, W, l, false);  // W is free until the identity is written below
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

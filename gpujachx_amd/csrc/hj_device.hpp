@@ -186,51 +186,10 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
   return __hiloint2double(hi, lo);
 }
 
-// In-place 32x32 Cholesky by one wave, share/device_code_common_Cholesky.hpp:58-158
-// (USE_RSQRT): lane x (lanes 32..63 duplicate lanes 0..31) holds row x of the
-// lower triangle, a[j] = A(x, j) for j <= x, else 0; column k is scaled by
-// rsqrt(A(k,k)), the diagonal set to sqrt(A(k,k)), then
-// A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k.  On return a[j] =
-// L(x, j).  The pivot comes by readlane; column k of L goes through the
-// 32-double LDS buffer col (broadcast reads, no scalar registers held).
-__device__ __forceinline__ void chol32_rows(double (&a)[32], unsigned x, double *col)
-{
-  // pivot of column 0; later pivots are taken as soon as their own update is
-  // done, so their rsqrt / sqrt overlap the rest of the column's updates
-  double akk = readlane_d(a[0], 0);
-  double rk = dev_rsqrt(akk), dk = sqrt(akk);
-#pragma unroll
-  for (unsigned k = 0u; k < 32u; ++k) {
-    if (x > k) a[k] = a[k] * rk;
-    else if (x == k) a[k] = dk;
-    if (k + 1u < 32u) {
-      // the LDS accesses of a wavefront complete in program order, and col
-      // may alias itself, so the compiler keeps write-then-read order
-      col[x] = a[k];
-      // the next pivot A(k+1,k+1) - L(k+1,k)^2 from lane k+1's own row
-      // (L(k+1,k) is its a[k]): the same fma as its column update below,
-      // without waiting for the column's round trip through LDS
-      akk = readlane_d(__fma_rn(-a[k], a[k], a[k + 1u]), k + 1u);
-      const double rn = dev_rsqrt(akk), dn = sqrt(akk);
-      {
-        const double l1 = col[k + 1u];
-        if (x >= k + 1u) a[k + 1u] = __fma_rn(-a[k], l1, a[k + 1u]);
-      }
-#pragma unroll
-      for (unsigned j = k + 2u; j < 32u; ++j) {
-        const double ljk = col[j];
-        if (x >= j) a[j] = __fma_rn(-a[k], ljk, a[j]);
-      }
-      rk = rn;
-      dk = dn;
-      // keep the next column's broadcast loads below this point (register
-      // pressure)
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-  }
-}
+// A double in LDS: helpers that take generic pointers to LDS (a caller's
+// dynamic LDS region, a function not inlined) access them through this type,
+// so the accesses are LDS instructions (ds_*) rather than flat ones.
+typedef __attribute__((address_space(3))) double lds_f64;
 
 // LDS of one wave's pair state: the 32x32 matrices R and W column-major with
 // leading dimension 33
@@ -243,6 +202,85 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// In-place 32x32 Cholesky by one wave, share/device_code_common_Cholesky.hpp:58-158
+// (USE_RSQRT): column k is scaled by rsqrt(A(k,k)), the diagonal set to
+// sqrt(A(k,k)), then A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for the rows
+// x >= j > k.  A: the Gram's lower triangle in LDS (A[c * LD + r] = A(r,c),
+// c <= r); on return A holds R = L^T (zeros below the diagonal).  col: 32
+// doubles of LDS for the column broadcast.
+//
+// Lane l keeps row x = l % 32 of the columns c = 2i + l/32 (i = 0..15), so
+// one instruction updates an even column in one half of the wave and an odd
+// one in the other: 16 registers and half the update instructions of a
+// row-per-lane layout.  The updates run on every row; above the diagonal
+// they produce values that no later step reads (the pivot is read on the
+// diagonal, the broadcast column below it) and that are not stored.  The
+// next pivot A(k+1,k+1) - L(k+1,k)^2 is formed from two readlanes as soon
+// as column k is scaled, so its rsqrt runs beside the column's updates.
+// check: whether every factor entry is finite and the diagonal positive
+// (wave-uniform).
+template <unsigned LD>
+__device__ __forceinline__ bool chol32_wave(double *A_, double *col_, unsigned l, bool check)
+{
+  lds_f64 *const A = (lds_f64 *)A_;
+  lds_f64 *const col = (lds_f64 *)col_;
+  const unsigned x = l & 31u, h = l >> 5;
+  double b[16];
+#pragma unroll
+  for (unsigned i = 0u; i < 16u; ++i) {
+    const unsigned c = 2u * i + h;
+    b[i] = (c <= x) ? A[c * LD + x] : 0.0;
+  }
+  double akk = readlane_d(b[0], 0);
+  double rk = dev_rsqrt(akk), dk = sqrt(akk);
+#pragma unroll
+  for (unsigned k = 0u; k < 32u; ++k) {
+    const unsigned ik = k >> 1, hk = k & 1u;
+    const bool own = (h == hk);
+    if (own && x > k) b[ik] = b[ik] * rk;
+    else if (own && x == k) b[ik] = dk;
+    if (k + 1u < 32u) {
+      const unsigned k1 = k + 1u;
+      if (own) col[x] = b[ik];
+      const double lk = readlane_d(b[ik], hk * 32u + k1);
+      const double d1 = readlane_d(b[k1 >> 1], (k1 & 1u) * 32u + k1);
+      akk = __fma_rn(-lk, lk, d1);
+      const double rn = dev_rsqrt(akk), dn = sqrt(akk);
+      // the other half reads what this half stored: without the fence the
+      // compiler may issue a lane's read of col[x] before the (conditional,
+      // other-lane) store, as no store of its own precedes it
+      wave_lds_sync();
+      const double lx = col[x];
+#pragma unroll
+      for (unsigned i = k1 >> 1; i < 16u; ++i) {
+        // column 2i + h; for i = k1/2 with k1 odd, only half 1's column is
+        // beyond k
+        if ((k1 & 1u) && i == (k1 >> 1) && h == 0u) continue;
+        b[i] = __fma_rn(-lx, col[2u * i + h], b[i]);
+      }
+      rk = rn;
+      dk = dn;
+      wave_lds_sync();
+    }
+  }
+  bool ok = true;
+  if (check) {
+#pragma unroll
+    for (unsigned i = 0u; i < 16u; ++i) {
+      const unsigned c = 2u * i + h;
+      if (c <= x) ok = ok && (fabs(b[i]) <= 0x1.fffffffffffffp+1023);
+      if (c == x) ok = ok && (b[i] > 0.0);
+    }
+    ok = (__ballot(ok) == ~0ull);
+  }
+#pragma unroll
+  for (unsigned i = 0u; i < 16u; ++i) {
+    const unsigned c = 2u * i + h;
+    A[x * LD + c] = (c <= x) ? b[i] : 0.0;
+  }
+  return ok;
 }
 
 // Sum of the 32 rows of one pivot pair held by 4 lanes, lane r holding rows
@@ -278,11 +316,13 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v)
 // and with COS the largest |A_pq| / sqrt(A_pp A_qq) over the pivots of the
 // first sweep.
 template <bool SORT, bool COS>
-__device__ __forceinline__ void wave_level0(double *R, double *W, const uint8_t *S0,
+__device__ __forceinline__ void wave_level0(double *R_, double *W_, const uint8_t *S0,
                                             unsigned s0steps, unsigned nswp, int def, bool hyp,
                                             unsigned npos, unsigned l, unsigned &out_s,
                                             unsigned &out_b, double &out_cos)
 {
+  lds_f64 *const R = (lds_f64 *)R_;
+  lds_f64 *const W = (lds_f64 *)W_;
   const unsigned y = l >> 2, r = l & 3u;
   unsigned blk_s = 0u, blk_b = 0u;
   double my_cos = 0.0;
@@ -348,13 +388,14 @@ __device__ __forceinline__ void wave_level0(double *R, double *W, const uint8_t 
         const unsigned PP = keep ? p : q, QQ = keep ? q : p;
 #pragma unroll
         for (unsigned k = 0u; k < 8u; ++k) {
-          // big rotation: scale by c; otherwise (c == 1) the fma result as is
+          // big rotation: scale by c; otherwise c == 1 exactly (dev_rot), and
+          // c * x is x
           const double rp = __fma_rn(t_, Gq[k], Gp[k]), rq = __fma_rn(t, Gp[k], Gq[k]);
           const double wp = __fma_rn(t_, Vq[k], Vp[k]), wq = __fma_rn(t, Vp[k], Vq[k]);
-          L0A(R, r + 4u * k, PP) = transf_b ? c * rp : rp;
-          L0A(R, r + 4u * k, QQ) = transf_b ? c * rq : rq;
-          L0A(W, r + 4u * k, PP) = transf_b ? c * wp : wp;
-          L0A(W, r + 4u * k, QQ) = transf_b ? c * wq : wq;
+          L0A(R, r + 4u * k, PP) = c * rp;
+          L0A(R, r + 4u * k, QQ) = c * rq;
+          L0A(W, r + 4u * k, PP) = c * wp;
+          L0A(W, r + 4u * k, QQ) = c * wq;
         }
       }
       else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {

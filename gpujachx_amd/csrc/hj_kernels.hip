@@ -420,29 +420,10 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
   // column k scaled by rsqrt(A(k,k)), diagonal sqrt(A(k,k)), then
   // A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) for x >= j > k; R = L^T.
   if (live && tid < 64u) {
-    const unsigned x = tid & 31u;
-    double a[32];
-#pragma unroll
-    for (unsigned j = 0u; j < 32u; ++j)
-      a[j] = (j <= x) ? RA(R, x, j) : 0.0;
-    chol32_rows(a, x, L.cholcol);
-    if (tid < 32u) {
-#pragma unroll
-      for (unsigned y = 0u; y < 32u; ++y)
-        RA(R, y, x) = (y <= x) ? a[y] : 0.0;
-    }
-    if (plan == PLAN_SPLIT) {
-      // a Gram from the split products may miss positive definiteness
-      // (nearly dependent columns): then no update, FP64 at the next visit
-      bool ok = true;
-#pragma unroll
-      for (unsigned j = 0u; j < 32u; ++j) {
-        ok = ok && (fabs(a[j]) <= 0x1.fffffffffffffp+1023);
-        if (j == x) ok = ok && (a[j] > 0.0);
-      }
-      const unsigned long long good = __ballot(ok);
-      if (tid == 0u) L.chol_ok = (good == ~0ull) ? 1 : 0;
-    }
+    // a Gram from the split products may miss positive definiteness (nearly
+    // dependent columns): then no update, FP64 at the next visit
+    const bool ok = chol32_wave<LDR>(R, L.cholcol, tid, plan == PLAN_SPLIT);
+    if (plan == PLAN_SPLIT && tid == 0u) L.chol_ok = ok ? 1 : 0;
   }
   __syncthreads();
   if (live && plan == PLAN_SPLIT && !L.chol_ok) {
@@ -533,13 +514,14 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
         const unsigned P = keep ? p : q, Q = keep ? q : p;
 #pragma unroll
         for (unsigned i = 0u; i < 4u; ++i) {
-          // big rotation: scale by c; otherwise (c == 1) the fma result as is
+          // big rotation: scale by c; otherwise c == 1 exactly (dev_rot), and
+          // c * x is x
           const double rp = __fma_rn(t_, Gq[i], Gp[i]), rq = __fma_rn(t, Gp[i], Gq[i]);
           const double wp = __fma_rn(t_, Vq[i], Vp[i]), wq = __fma_rn(t, Vp[i], Vq[i]);
-          RA(R, r + 8u * i, P) = transf_b ? c * rp : rp;
-          RA(R, r + 8u * i, Q) = transf_b ? c * rq : rq;
-          RA(W, r + 8u * i, P) = transf_b ? c * wp : wp;
-          RA(W, r + 8u * i, Q) = transf_b ? c * wq : wq;
+          RA(R, r + 8u * i, P) = c * rp;
+          RA(R, r + 8u * i, Q) = c * rq;
+          RA(W, r + 8u * i, P) = c * wp;
+          RA(W, r + 8u * i, Q) = c * wq;
         }
       }
       else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
