@@ -8,7 +8,7 @@ family come from the library's autotuner (on a sample of the batch).  A run
 restores the inputs, then times hj_batch_run; reports problems/s and the
 p50/p99 of the per-problem device time and completion time.
 
-    python -m gpujachx_amd.batch_bench [--count 4096] [--runs 2] [--no-tune]
+    python -m gpujachx_amd.batch_bench [--count 4096] [--runs 2] [--no-tune | --cfg W,G,S,L]
 """
 import argparse
 import json
@@ -62,6 +62,53 @@ def check_sample(torch, probs, k):
     return worst_j, worst_r
 
 
+FP64_PEAK_TFS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def visits_per_sweep(g, n1, strategy):
+    """Block-pair visits of one level-2 sweep of order n1 under the batch's
+    strategy choice (the preferred family where it has a construction, else
+    BrentL, else mmstep): steps x n1/2."""
+    pref = {0: "cycwor", 1: "BrentL", 2: "mmstep"}[int(strategy)]
+    for fam in (pref, "BrentL", "mmstep"):
+        try:
+            return g.strat_generate(fam, n1).shape[0] * (n1 // 2)
+        except ValueError:
+            continue
+    raise ValueError("no strategy of order %d" % n1)
+
+
+def roofline(g, probs, sweeps, strategy, alg, seconds):
+    """Algorithmic work of the batch and its rates.  Per block-pair visit (32
+    columns, m rows, n_V = n rows of V when accumulated): Gram 2048 m flops,
+    update 2048 (m + n_V); bytes 256 (3 m + 2 n_V) (Gram read, update read and
+    write of the pair, V read and write), times the visits of every executed
+    sweep."""
+    flops = bytes_ = 0.0
+    visits = 0
+    cache = {}
+    for p, sw in zip(probs, sweeps):
+        n1 = p["n"] // 16
+        if n1 not in cache:
+            cache[n1] = visits_per_sweep(g, n1, strategy)
+        v = cache[n1] * int(sw)
+        nv = p["n"] if alg & 2 else 0
+        visits += v
+        flops += v * 2048.0 * (2.0 * p["m"] + nv)
+        bytes_ += v * 256.0 * (3.0 * p["m"] + 2.0 * nv)
+    tf = flops / seconds / 1e12
+    gb = bytes_ / seconds / 1e9
+    return {"algorithmic_flops": flops, "algorithmic_bytes": bytes_, "visits": visits,
+            "tflops": tf, "frac_fp64_peak": tf / FP64_PEAK_TFS, "GBs": gb,
+            "frac_hbm_peak": gb / HBM_PEAK_GBS,
+            "peaks": "FP64 MFMA 78.6 TFLOP/s, HBM 8 TB/s (MI355X published)",
+            "bound": "hbm",
+            "note": "counter runs of the same workload (profiles/roofline_batch_4096.md): measured "
+                    "HBM traffic 0.96x the algorithmic bytes at 4.65 TB/s, FP64 MFMA busy 28 %: "
+                    "bound by HBM, the visits' latency taking the rest"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=4096)
@@ -71,6 +118,8 @@ def main():
     ap.add_argument("--sample", type=int, default=256, help="autotuner sample size")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--max-sweeps", type=int, default=30)
+    ap.add_argument("--cfg", default=None,
+                    help="launch shape waves,wg_per_cu,strategy,longest_first instead of tuning")
     args = ap.parse_args()
     os.environ.setdefault("HJ_QUIET", "1")
     import torch
@@ -84,9 +133,12 @@ def main():
     gen_s = time.time() - t0
     descs = descs_of(g, probs, args.alg)
     tune = None
-    if args.no_tune:
+    if args.no_tune or args.cfg:
         cfg = g.default_batch_cfg()
         cfg.max_sweeps = args.max_sweeps
+        if args.cfg:
+            cfg.waves, cfg.wg_per_cu, cfg.strategy, cfg.longest_first = \
+                [int(x) for x in args.cfg.split(",")]
     else:
         t0 = time.time()
         cfg, rate = g.batch_autotune(descs, sample=args.sample, max_sweeps=args.max_sweeps)
@@ -119,6 +171,7 @@ def main():
         "sweeps": {"mean": float(res["sweeps"].mean()), "max": int(res["sweeps"].max()),
                    "at_cap": int((res["sweeps"] >= cfg.max_sweeps).sum())},
         "cfg": cfg.as_dict(), "autotune": tune,
+        "roofline": roofline(g, probs, res["sweeps"], cfg.strategy, args.alg, best),
         "check_32": {"VtJV_minus_J_max": ej, "residual_rel_max": er},
         "workload": "N uniform in {64..1024} step 64, M in {N, 2N}, half SVD (NPLUS=N), half HSVD "
                     "(NPLUS=N/2), G iid N(0,1) x 10^U(-3,3) column scales, FP64",
