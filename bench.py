@@ -339,15 +339,27 @@ def main():
     tfile = os.path.join(ROOT, "gpujachx_amd", "data", "upgram_traffic.json")
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
-        key = "k_stage" if staged_used and "k_stage" in tj["kernels"] else "k_upgram"
-        up = tj["kernels"][key]
-        traffic = up["bytes_per_visit"] * visits_launch * n / tj["n"]
-        traffic_src = {"file": "gpujachx_amd/data/upgram_traffic.json", "kernel": key,
-                       "bytes_per_visit_at_n": [up["bytes_per_visit"], tj["n"]],
-                       "traffic_over_fused_minimum": up["traffic_over_algorithmic"],
-                       "counter_scale": up.get("counter_scale",
-                                               [tj["calibration"]["fetch_scale"],
-                                                tj["calibration"]["write_scale"]])}
+        run = tj.get("runs", {}).get(args.form)
+        if run is not None and (staged_used or flow_used):
+            # a counter run of this form's whole SVD (pmc_learned): its bytes
+            # per executed visit, times this run's visits per launch
+            traffic = run["bytes_per_visit"] * visits_launch * n / tj["n"]
+            traffic_src = {"file": "gpujachx_amd/data/upgram_traffic.json",
+                           "measured": "whole %s SVD at N=%d (%d sweeps, %d executed visits)"
+                                       % (args.form, tj["n"], run["sweeps"], run["visits_executed"]),
+                           "bytes_per_visit_at_n": [run["bytes_per_visit"], tj["n"]],
+                           "traffic_over_algorithmic": run["traffic_over_algorithmic"],
+                           "counter_scale": run["counter_scale"]}
+        else:
+            key = "k_stage" if staged_used and "k_stage" in tj["kernels"] else "k_upgram"
+            up = tj["kernels"][key]
+            traffic = up["bytes_per_visit"] * visits_launch * n / tj["n"]
+            traffic_src = {"file": "gpujachx_amd/data/upgram_traffic.json", "kernel": key,
+                           "bytes_per_visit_at_n": [up["bytes_per_visit"], tj["n"]],
+                           "traffic_over_fused_minimum": up["traffic_over_algorithmic"],
+                           "counter_scale": up.get("counter_scale",
+                                                   [tj["calibration"]["fetch_scale"],
+                                                    tj["calibration"]["write_scale"]])}
 
     def kstat(k, bytes_per_visit=None, flops_per_visit=None):
         if kcnt[k] <= 0 or kms[k] <= 0:

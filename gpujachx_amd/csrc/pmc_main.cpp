@@ -9,8 +9,15 @@
 // times column scales 10^U(-3,3); prints the batch's algorithmic flops and
 // bytes (from the sweeps each problem ran) and its time.
 //
+// Learned mode: the calibration copy, then a whole SVD (ALG 9, cycwor,
+// learned skip + split Gram, at most 80 sweeps) of an N x N column-major
+// float64 matrix read from a file, with the skip model's float32
+// parameters read from another (gpujachx_amd/pmc_learned.py writes both);
+// prints the run's sweeps and executed / skipped visits as JSON.
+//
 //   hj_pmc N K [COPY_MIB]
 //   hj_pmc batch COUNT WAVES WG_PER_CU STRATEGY LONGEST_FIRST [COPY_MIB]
+//   hj_pmc learned N G_FILE MODEL_FILE THRESHOLD MAX_SKIPS [COPY_MIB]
 //   rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc -- gpujachx_amd/lib/hj_pmc 8192 16
 #include <hip/hip_runtime.h>
 
@@ -146,10 +153,81 @@ static int batch_main(int argc, char **argv)
   return rc ? 1 : 0;
 }
 
+static int learned_main(int argc, char **argv)
+{
+  if (argc < 7) {
+    (void)fprintf(stderr, "%s learned N G_FILE MODEL_FILE THRESHOLD MAX_SKIPS [COPY_MIB]\n", argv[0]);
+    return 2;
+  }
+  const unsigned n = (unsigned)atoi(argv[2]);
+  const float thr = (float)atof(argv[5]);
+  const unsigned max_skips = (unsigned)atoi(argv[6]);
+  const size_t copy_mib = (argc > 7) ? (size_t)atol(argv[7]) : 1024u;
+  if (!n || (n % 64u)) {
+    (void)fprintf(stderr, "N must be a positive multiple of 64\n");
+    return 2;
+  }
+  std::vector<double> h((size_t)n * n);
+  std::vector<float> model;
+  {
+    FILE *f = fopen(argv[3], "rb");
+    if (!f || fread(h.data(), sizeof(double), h.size(), f) != h.size()) {
+      (void)fprintf(stderr, "cannot read %s\n", argv[3]);
+      return 1;
+    }
+    (void)fclose(f);
+    f = fopen(argv[4], "rb");
+    if (!f) {
+      (void)fprintf(stderr, "cannot read %s\n", argv[4]);
+      return 1;
+    }
+    float v;
+    while (fread(&v, sizeof(float), 1u, f) == 1u)
+      model.push_back(v);
+    (void)fclose(f);
+  }
+  if (configureGPU(0) < 0)
+    return 1;
+  if (calibration_copy(copy_mib))
+    return 1;
+  hj_init_strats_builtin("cycwor", 32u, "cycwor", n / 16u);
+  if (hj_set_skip_model(model.data(), (unsigned)model.size(), thr, max_skips)) {
+    (void)fprintf(stderr, "skip model: wrong parameter count %zu\n", model.size());
+    return 1;
+  }
+  hj_set_form(HJ_FORM_LEARNED_SKIP | HJ_FORM_SPLIT_GRAM);
+  const unsigned cap = hj_set_max_sweeps(80u);
+  double *dG = nullptr, *dD = nullptr;
+  CHECK(hipMalloc(&dG, sizeof(double) * h.size()));
+  CHECK(hipMalloc(&dD, sizeof(double) * n));
+  CHECK(hipMemcpy(dG, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  unsigned sw = 0u;
+  unsigned long long gs = 0ull, gb = 0ull;
+  const int rc = hypJacL2_dev(9u, n, n, n, dG, n, nullptr, 0u, dD, &sw, &gs, &gb, nullptr, nullptr);
+  CHECK(hipDeviceSynchronize());
+  hj_set_form(0u);
+  (void)hj_set_max_sweeps(cap);
+  unsigned per[256];
+  unsigned long long skipped = 0ull, visits = 0ull, splits = 0ull;
+  hj_last_skips(per, &skipped, &visits);
+  hj_last_splits(per, &splits);
+  unsigned long long launches = 0ull;
+  (void)hj_kernel_ms(5, &launches);
+  (void)printf("{\"n\": %u, \"form\": \"learned+split\", \"ret\": %d, \"sweeps\": %u, "
+               "\"visits_executed\": %llu, \"visits_skipped\": %llu, \"split_grams\": %llu, "
+               "\"staged\": %d, \"staged_launches\": %llu, \"copy_mib\": %zu}\n",
+               n, rc, sw, visits, skipped, splits, hj_last_staged(), launches, copy_mib);
+  CHECK(hipFree(dG));
+  CHECK(hipFree(dD));
+  return rc ? 1 : 0;
+}
+
 int main(int argc, char **argv)
 {
   if (argc >= 2 && !strcmp(argv[1], "batch"))
     return batch_main(argc, argv);
+  if (argc >= 2 && !strcmp(argv[1], "learned"))
+    return learned_main(argc, argv);
   if (argc < 3) {
     (void)fprintf(stderr, "%s N K [COPY_MIB]\n", argv[0]);
     return 2;

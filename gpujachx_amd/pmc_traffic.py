@@ -10,6 +10,7 @@ fixed factor on this GPU is corrected by that factor.
         -- <repo>/gpujachx_amd/lib/hj_pmc 8192 8
     (the same with WRITE_SIZE)
     python -m gpujachx_amd.pmc_traffic FETCH.csv WRITE.csv [--copy-mib 1024] [--out FILE]
+    python -m gpujachx_amd.pmc_traffic FETCH.csv WRITE.csv --run-json RUN.json   (pmc_learned)
 
 Writes a JSON file with bytes per launch and per executed pair visit
 (N x 32 columns read and written) for k_upgram, k_update and k_gram; with
@@ -41,6 +42,43 @@ def per_kernel(path):
     return out
 
 
+def run_bytes(path, scale):
+    """Counter total (KiB x scale) over the sweeps' kernels of a whole run
+    (every hj:: dispatch but the final column normalisation), and the
+    k_stage dispatches' values."""
+    tot, stage = 0.0, []
+    for r in csv.DictReader(open(path)):
+        name = r["Kernel_Name"]
+        if "hj::" not in name or "k_initD" in name:
+            continue
+        v = float(r["Counter_Value"]) * scale * 1024.0
+        tot += v
+        if "k_stage<" in name:
+            stage.append(v)
+    return tot, stage
+
+
+def whole_run(args, fs, ws):
+    run = json.loads(open(args.run_json).read().strip().splitlines()[-1])
+    fb, fstage = run_bytes(args.fetch_csv, fs)
+    wb, wstage = run_bytes(args.write_csv, ws)
+    v = run["visits_executed"]
+    ent = {"form": run["form"], "sweeps": run["sweeps"], "visits_executed": v,
+           "visits_skipped": run["visits_skipped"], "fetch_bytes": fb, "write_bytes": wb,
+           "bytes_per_visit": (fb + wb) / v,
+           "traffic_over_algorithmic": (fb + wb) / v / (768.0 * run["n"]),
+           "stage_dispatches": len(fstage),
+           "stage_bytes_per_dispatch": (sum(fstage) + sum(wstage)) / max(len(fstage), 1),
+           "counter_scale": [fs, ws],
+           "note": "every hj:: dispatch of the run but the final normalisation, over the run's "
+                   "executed visits; algorithmic 768 m B per visit"}
+    out = json.load(open(args.out)) if os.path.exists(args.out) else {"n": run["n"], "kernels": {}}
+    out.setdefault("runs", {})[run["form"]] = ent
+    with open(args.out, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(ent))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("fetch_csv")
@@ -49,12 +87,17 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--merge", action="store_true",
                     help="keep the other kernels' entries of an existing output file")
+    ap.add_argument("--run-json", default=None,
+                    help="the JSON line of a whole-run driver (pmc_learned): record the run's "
+                         "bytes per executed visit under its form instead of per-launch figures")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                   "upgram_traffic.json"))
     args = ap.parse_args(argv)
     f, w = per_kernel(args.fetch_csv), per_kernel(args.write_csv)
     copy_kib = args.copy_mib * 1024.0
     fs, ws = copy_kib / f["copy"], copy_kib / w["copy"]
+    if args.run_json:
+        return whole_run(args, fs, ws)
     visits = args.n // 32            # pairs per step when none is skipped
     res = {"n": args.n, "visits_per_launch": visits,
            "calibration": {"copy_mib": args.copy_mib, "fetch_kib": f["copy"], "write_kib": w["copy"],
