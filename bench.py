@@ -166,8 +166,6 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the profiled run (HIP events around every launch) that gives the "
                          "per-kernel breakdown")
-    ap.add_argument("--flow", choices=["on", "off"], default="off",
-                    help="dataflow sweeps (one persistent launch per sweep) for both forms")
     ap.add_argument("--staged", choices=["on", "off"], default="on",
                     help="staged sweeps (pair visits overlapped with half-step block updates)")
     ap.add_argument("--model", default=None, help="skip-model file (default: the packaged one)")
@@ -190,7 +188,6 @@ def main():
     g.configureGPU(0)
     g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
     g.set_max_sweeps(args.max_sweeps)
-    g.set_dataflow(args.flow == "on")
     g.set_staged(args.staged == "on")
     L = g.lib()
     G0, s_exact = make_input(torch, n, args.seed)
@@ -254,7 +251,7 @@ def main():
         print("bench: %s nplus=%d %d run(s) %.3f s each, %d sweeps" % (form, nplus, steps,
               elapsed / steps, sw.value), file=sys.stderr, flush=True)
         return {"per": elapsed / steps, "kms": kms, "kcnt": kcnt, "visits": visits,
-                "flow": g.last_dataflow(), "staged": g.last_staged(),
+                "staged": g.last_staged(),
                 "sweeps": sw.value, "rot_b": rot_b, "skipped": skips[1], "split": splits[1],
                 "split_per_sweep": splits[0], "skip_per_sweep": skips[0], "D": Dh,
                 "sigma": np.sqrt(np.abs(Dh))}
@@ -313,18 +310,16 @@ def main():
     # time (HIP events around each graph launch on the library's stream)
     # over their staged launches.
     staged_used = bool(main_run["staged"])
-    flow_used = bool(main_run["flow"])
     nl = None
-    if staged_used or flow_used:
+    if staged_used:
         nl = max(main_run["kcnt"][5], 1)
         ms_launch = main_run["kms"][5] / nl
         visits_launch = main_run["visits"] / nl
         timing_src = ("the timed region: HIP events around each sweep's graph launch, the sweeps' "
                       "GPU time over their %s launches (step 0's Gram and visits, the plan and the "
-                      "last update, about 1 %% of a sweep, included)"
-                      % ("k_stage" if staged_used else "k_flow"))
+                      "last update, about 1 %% of a sweep, included)" % "k_stage")
         kname = ("k_stage (staged sweep: half a step's fused block updates + next-step Grams "
-                 "beside pair visits)") if staged_used else "k_flow (dataflow sweep)"
+                 "beside pair visits)")
     else:
         ms_launch = kms[2] / max(kcnt[2], 1)
         visits_launch = pk["visits"] / max(kcnt[2], 1)
@@ -340,7 +335,7 @@ def main():
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
         run = tj.get("runs", {}).get(args.form)
-        if run is not None and (staged_used or flow_used):
+        if run is not None and staged_used:
             # a counter run of this form's whole SVD (pmc_learned): its bytes
             # per executed visit, times this run's visits per launch
             traffic = run["bytes_per_visit"] * visits_launch * n / tj["n"]
@@ -376,7 +371,7 @@ def main():
     kernels = {"source": "profiled run (HIP events around every launch)" if prof_run is not None
                else "the timed run"}
     for name, k, bpv, fpv in (("gram", 0, 256.0, 1536.0), ("pair", 1, None, None),
-                              ("update", 2, 512.0, 3584.0), ("flow", 3, None, None),
+                              ("update", 2, 512.0, 3584.0),
                               ("stage", 4, 512.0, 3584.0)):
         st = kstat(k, bpv, fpv)
         if st is not None:
@@ -428,7 +423,6 @@ def main():
                      "mfma_frac_fp64": (flops / (ms_launch * 1e-3) / 1e12 / FP64_PEAK_TFS)
                                        if ms_launch > 0 else None},
         "kernels": kernels,
-        "dataflow": flow_used,
         "staged": staged_used,
         "profiled_s": ({"note": "the same SVD launched step by step with HIP events around "
                                 "every launch (the per-kernel breakdown's run); `value` is the "

@@ -92,12 +92,8 @@ def lib():
     L.hj_set_max_sweeps.restype = u
     L.hj_set_step_limit.argtypes = [ctypes.c_long]
     L.hj_set_step_limit.restype = None
-    L.hj_set_dataflow.argtypes = [i]
-    L.hj_set_dataflow.restype = i
     L.hj_set_factorization.argtypes = [i]
     L.hj_set_factorization.restype = i
-    L.hj_last_dataflow.argtypes = []
-    L.hj_last_dataflow.restype = i
     L.hj_set_staged.argtypes = [i]
     L.hj_set_staged.restype = i
     L.hj_last_staged.argtypes = []
@@ -271,12 +267,6 @@ def set_form(form):
     lib().hj_set_form(int(form))
 
 
-def set_dataflow(on):
-    """Dataflow sweeps on/off (one persistent launch per sweep where the
-    strategy and routine allow it); returns the previous setting."""
-    return bool(lib().hj_set_dataflow(1 if on else 0))
-
-
 FACT_CHOLESKY, FACT_QR, FACT_QR_DRMAC = 0, 1, 2
 
 
@@ -285,11 +275,6 @@ def set_factorization(f):
     FACT_QR_DRMAC (the reference's build options USE_QR / USE_DRMAC);
     returns the previous setting."""
     return lib().hj_set_factorization(int(f))
-
-
-def last_dataflow():
-    """Whether the last hypJacL2 / hypJacL2_dev call ran dataflow sweeps."""
-    return bool(lib().hj_last_dataflow())
 
 
 STAGED_OFF, STAGED_AUTO, STAGED_ALWAYS = 0, 1, 2

@@ -1,6 +1,6 @@
 // hj_fused.hpp -- wave-level pieces of the fused step transition (the block
 // update of a closed quad's two pairs feeding the next step's Gram from
-// registers), shared by k_upgram / k_flow (hj_kernels.hip) and the batched
+// registers), shared by k_upgram / k_stage (hj_kernels.hip) and the batched
 // launch (hj_batch.hip).
 //
 // Update orientation D = G W (A = G, B = W): lane l, register r of the

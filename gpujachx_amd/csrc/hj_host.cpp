@@ -56,9 +56,6 @@ static unsigned g_skip_min_visits = 32000u;
 // factorization of the block pairs: HJ_FACT_CHOLESKY (Gram + Cholesky),
 // HJ_FACT_QR, HJ_FACT_QR_DRMAC
 static int g_fact = HJ_FACT_CHOLESKY;
-// dataflow sweeps (one persistent launch per sweep) where the strategy allows
-static int g_flow = 0;
-static int g_last_flow = 0;  // the last run used them
 // staged sweeps (k_stage): 0 off, 1 where a step's block updates move at
 // least STAGE_MIN_BYTES, 2 wherever they apply
 static int g_stage = 1;
@@ -173,19 +170,6 @@ struct Work {
   Quad *d_quads = nullptr;     // [steps - 1][np / 2] closed quads of each step transition
   std::vector<char> fuse;      // transition s -> s+1 runs as one fused update + Gram
   std::vector<Quad> hquads;
-  // dataflow sweep (k_flow): item list, rings, counters
-  static constexpr unsigned FLOW_RING = 3u;
-  // fused items listed between a quad's own and its next pairs' visit item
-  static constexpr unsigned FLOW_DELAY = 0u;
-  bool flow_ok = false;
-  unsigned nitems = 0u, flow_blocks = 0u;
-  FlowItem *d_items = nullptr;
-  uint16_t *d_srcq = nullptr;
-  double *d_fpart = nullptr, *d_fW = nullptr;
-  unsigned *d_fflags = nullptr, *d_fcnt = nullptr;
-  size_t ncnt = 0u;
-  unsigned long long *d_trace = nullptr;  // item timestamps (HJ_FLOW_TRACE), else null
-  std::vector<FlowItem> hitems;
   // staged sweep: per transition the next pairs of quads [0, nq/2) then of
   // the rest; transitions whose first-half quads need a pair of the previous
   // transition's second half
@@ -303,138 +287,6 @@ struct Work {
     return true;
   }
 
-  // The item list of a dataflow sweep: the visits of step 0 (two pairs per
-  // item), then per transition s -> s+1 the fused items of each quad (all
-  // row chunks), quads in the order their pairs' visits were listed, and the
-  // visit item of the quad's two next pairs `delay` fused items after its
-  // own (so that its inputs are mostly done when a workgroup takes it), the
-  // rest at the end of the transition.  Every item comes after the items it
-  // waits for.
-  bool build_flow()
-  {
-    flow_ok = false;
-    if (hquads.empty() || steps < 2u) return false;
-    for (unsigned t = 0u; t + 1u < steps; ++t)
-      if (!fuse[t]) return false;
-    const unsigned nq = np / 2u;
-    std::vector<FlowItem> items;
-    std::vector<unsigned> pos(np), npos(np);
-    std::vector<uint16_t> srcq((size_t)steps * np, 0u);
-    auto emit_pairs = [&](unsigned st, const uint16_t *p, unsigned k, std::vector<unsigned> &ps) {
-      FlowItem it;
-      it.kind = FLOW_PAIRS;
-      it.step = (uint16_t)st;
-      for (unsigned i = 0u; i < 4u; ++i) it.a[i] = (i < k) ? p[i] : FLOW_NONE;
-      for (unsigned i = 0u; i < k; ++i) ps[p[i]] = (unsigned)items.size();
-      items.push_back(it);
-    };
-    for (unsigned p = 0u; p < np; p += 2u) {
-      uint16_t g[2];
-      const unsigned k = std::min(2u, np - p);
-      for (unsigned i = 0u; i < k; ++i) g[i] = (uint16_t)(p + i);
-      emit_pairs(0u, g, k, pos);
-    }
-    const char *dl = getenv("HJ_FLOW_DELAY");
-    const unsigned delay = dl ? (unsigned)atoi(dl) : FLOW_DELAY;
-    std::vector<unsigned> order(nq);
-    for (unsigned t = 0u; t + 1u < steps; ++t) {
-      const Quad *Q = hquads.data() + (size_t)t * nq;
-      for (unsigned q = 0u; q < nq; ++q) {
-        order[q] = q;
-        srcq[(size_t)(t + 1u) * np + Q[q].n0] = (uint16_t)q;
-        srcq[(size_t)(t + 1u) * np + Q[q].n1] = (uint16_t)q;
-      }
-      std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned b) {
-        return std::max(pos[Q[a].p0], pos[Q[a].p1]) < std::max(pos[Q[b].p0], pos[Q[b].p1]);
-      });
-      // pending visit items: (fused items listed when they may go, pairs)
-      std::vector<std::pair<size_t, std::array<uint16_t, 2>>> pend;
-      size_t head = 0u, nupg = 0u;
-      for (unsigned q : order) {
-        for (unsigned c = 0u; c < nchunk; ++c) {
-          FlowItem it;
-          it.kind = FLOW_UPGRAM;
-          it.step = (uint16_t)t;
-          it.a[0] = (uint16_t)q;
-          it.a[1] = (uint16_t)c;
-          it.a[2] = it.a[3] = 0u;
-          items.push_back(it);
-          ++nupg;
-          while (head < pend.size() && pend[head].first <= nupg) {
-            emit_pairs(t + 1u, pend[head].second.data(), 2u, npos);
-            ++head;
-          }
-        }
-        pend.push_back({nupg + delay, {Q[q].n0, Q[q].n1}});
-        while (head < pend.size() && pend[head].first <= nupg) {
-          emit_pairs(t + 1u, pend[head].second.data(), 2u, npos);
-          ++head;
-        }
-      }
-      for (; head < pend.size(); ++head)
-        emit_pairs(t + 1u, pend[head].second.data(), 2u, npos);
-      pos.swap(npos);
-    }
-    nitems = (unsigned)items.size();
-    if (getenv("HJ_FLOW_TRACE")) {
-      HIP_CALL(hipMalloc(&d_trace, sizeof(unsigned long long) * 3u * items.size()));
-      hitems = items;
-    }
-    HIP_CALL(hipMalloc(&d_items, sizeof(FlowItem) * items.size()));
-    HIP_CALL(hipMemcpy(d_items, items.data(), sizeof(FlowItem) * items.size(), hipMemcpyHostToDevice));
-    HIP_CALL(hipMalloc(&d_srcq, sizeof(uint16_t) * srcq.size()));
-    HIP_CALL(hipMemcpy(d_srcq, srcq.data(), sizeof(uint16_t) * srcq.size(), hipMemcpyHostToDevice));
-    HIP_CALL(hipMalloc(&d_fpart, sizeof(double) * 768u * nchunk * np * FLOW_RING));
-    HIP_CALL(hipMalloc(&d_fW, sizeof(double) * 1024u * np * FLOW_RING));
-    HIP_CALL(hipMalloc(&d_fflags, sizeof(unsigned) * np * FLOW_RING));
-    ncnt = 64u + (size_t)steps * (nq + np + 2u);
-    HIP_CALL(hipMalloc(&d_fcnt, sizeof(unsigned) * ncnt));
-    int dev = 0, ncu = 0;
-    HIP_CALL(hipGetDevice(&dev));
-    HIP_CALL(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const unsigned cap = flow_blocks_per_cu() * (unsigned)std::max(ncu, 1);
-    flow_blocks = std::min(nitems, cap);
-    flow_ok = true;
-    return true;
-  }
-
-  FlowArgs flow_args(double *dG, unsigned lddG, bool planned, unsigned nswp, int definite,
-                     unsigned nplus) const
-  {
-    FlowArgs a;
-    const unsigned nq = np / 2u;
-    a.G = dG;
-    a.ldG = lddG;
-    a.np = np;
-    a.nq = nq;
-    a.nchunk = nchunk;
-    a.CH = CH;
-    a.nw = gram_waves(CH);
-    a.R = FLOW_RING;
-    a.nitems = nitems;
-    a.s1 = d_s1;
-    a.quads = d_quads;
-    a.srcq = d_srcq;
-    a.items = d_items;
-    a.part = d_fpart;
-    a.W = d_fW;
-    a.flags = d_fflags;
-    a.ctl = d_ctl;
-    a.rec = d_rec;
-    a.plan = planned ? d_skip : nullptr;
-    a.strat0 = d_s0;
-    a.s0steps = s0steps;
-    a.nswp = nswp;
-    a.definite = definite;
-    a.nplus = nplus;
-    a.cnt = d_fcnt;
-    a.qdone = d_fcnt + 64u;
-    a.pdone = a.qdone + (size_t)steps * nq;
-    a.supg = a.pdone + (size_t)steps * np;
-    a.spg = a.supg + steps;
-    a.trace = d_trace;
-    return a;
-  }
   void release()
   {
     HIP_CALL(hipFree(d_s1));
@@ -451,26 +303,6 @@ struct Work {
     if (d_plist) HIP_CALL(hipFree(d_plist));
     if (d_plist_run) HIP_CALL(hipFree(d_plist_run));
     if (d_qord) HIP_CALL(hipFree(d_qord));
-    if (d_items) HIP_CALL(hipFree(d_items));
-    if (d_srcq) HIP_CALL(hipFree(d_srcq));
-    if (d_fpart) HIP_CALL(hipFree(d_fpart));
-    if (d_fW) HIP_CALL(hipFree(d_fW));
-    if (d_fflags) HIP_CALL(hipFree(d_fflags));
-    if (d_fcnt) HIP_CALL(hipFree(d_fcnt));
-    if (d_trace) {
-      // diagnostic: the item list and the timestamps of the last dataflow
-      // sweep to the file HJ_FLOW_TRACE names (u32 count, then per item
-      // kind, step, a[4] as u16 and three u64 clocks of 100 MHz)
-      std::vector<unsigned long long> t((size_t)3u * nitems);
-      HIP_CALL(hipMemcpy(t.data(), d_trace, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
-      if (FILE *f = fopen(getenv("HJ_FLOW_TRACE"), "wb")) {
-        (void)fwrite(&nitems, sizeof(unsigned), 1u, f);
-        (void)fwrite(hitems.data(), sizeof(FlowItem), hitems.size(), f);
-        (void)fwrite(t.data(), sizeof(unsigned long long), t.size(), f);
-        (void)fclose(f);
-      }
-      HIP_CALL(hipFree(d_trace));
-    }
   }
 };
 
@@ -569,27 +401,6 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
     have_gram = fused;
   }
   return issued;
-}
-
-// One level-2 sweep as a dataflow launch: the counters to zero, the Gram of
-// step 0 into ring slot 0, k_flow (every pair visit and every fused
-// transition), then the plain update of the last step.  Profiling events:
-// before and after k_flow.
-static void issue_sweep_flow(hipStream_t s, const Work &w, bool sort, unsigned nswp, int definite,
-                             unsigned nplus, double *dG, unsigned lddG, unsigned nrow,
-                             hipEvent_t *ev, bool planned, bool split)
-{
-  launch_flow_reset(s, w.d_fcnt, w.ncnt);
-  const uint8_t *plan0 = planned ? w.d_skip : nullptr;
-  launch_gram(s, dG, lddG, w.d_s1, w.np, w.nchunk, w.CH, w.d_fpart, plan0);
-  if (split)
-    launch_gram_split(s, dG, lddG, w.d_s1, w.np, w.nchunk, w.CH, w.d_fpart, plan0);
-  if (ev) HIP_CALL(hipEventRecord(ev[0], s));
-  launch_flow(s, sort, w.flow_args(dG, lddG, planned, nswp, definite, nplus), w.flow_blocks);
-  if (ev) HIP_CALL(hipEventRecord(ev[1], s));
-  const unsigned last = w.steps - 1u, ring = last % Work::FLOW_RING;
-  launch_update(s, dG, lddG, nrow, w.chG, nullptr, 0u, 0u, w.chV, w.d_s1 + (size_t)last * w.np * 2u,
-                w.np, w.d_fW + (size_t)ring * w.np * 1024u, w.d_fflags + (size_t)ring * w.np);
 }
 
 // One level-2 sweep as staged launches.  With A_t / B_t the first / second
@@ -749,11 +560,8 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
 
   Work w;
   w.alloc(nrow, ncol, full_svd);
-  const bool flow = g_flow && !full_svd && g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY &&
-                    w.build_flow();
-  g_last_flow = flow ? 1 : 0;
   const double step_bytes = 512.0 * (ncol / 32u) * ((double)nrow + (full_svd ? ncol : 0u));
-  const bool staged = !flow && g_stage && (g_stage == 2 || step_bytes >= STAGE_MIN_BYTES) &&
+  const bool staged = g_stage && (g_stage == 2 || step_bytes >= STAGE_MIN_BYTES) &&
                       g_step_limit <= 0 && g_fact == HJ_FACT_CHOLESKY && w.build_stage();
   g_last_stage = staged ? 1 : 0;
   if (full_svd)
@@ -792,7 +600,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   hipGraphExec_t gexec = nullptr;
   // graph path: events around each sweep's graph launch, and the launches
   // of the sweep's dominant kind it holds (staged launches; else the steps'
-  // update-class launches; one dataflow launch)
+  // update-class launches)
   hipEvent_t gev[2] = {nullptr, nullptr};
   unsigned long long graph_launches = 0ull;
   std::vector<hipEvent_t> evs;
@@ -812,11 +620,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     hipGraph_t graph;
     HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     sweep_head(s);
-    if (flow) {
-      issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, nullptr, planned, split);
-      graph_launches = 1ull;
-    }
-    else if (staged)
+    if (staged)
       graph_launches = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V,
                                           lddV, ncol, nullptr, planned, split, learned) - 2u;
     else
@@ -843,12 +647,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     if (eager) {
       sweep_head(s);
       long left = (g_step_limit > 0) ? (g_step_limit - steps_done) : (long)w.steps;
-      if (flow) {
-        issue_sweep_flow(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow,
-                         g_profile ? evs.data() : nullptr, planned, split);
-        issued = w.steps;
-      }
-      else if (staged) {
+      if (staged) {
         nstage_ev = issue_sweep_staged(s, w, cdsort, nswp, definite, nplus, dG, lddG, nrow, V,
                                        lddV, ncol, g_profile ? evs.data() : nullptr, planned,
                                        split, learned);
@@ -875,13 +674,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       g_stats.kms[5] += ms;
       g_stats.kcnt[5] += graph_launches;
     }
-    if (g_profile && flow) {
-      float ms = 0.0f;
-      HIP_CALL(hipEventElapsedTime(&ms, evs[0], evs[1]));
-      g_stats.kms[3] += ms;
-      g_stats.kcnt[3] += 1ull;
-    }
-    else if (g_profile && staged) {
+    if (g_profile && staged) {
       // kind 4: every launch after step 0's Gram and visits
       for (unsigned e = 1u; e + 1u < nstage_ev; ++e) {
         float ms = 0.0f;
@@ -900,8 +693,6 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
           g_stats.kcnt[k] += 1ull;
         }
     ++blk_swp;
-    if (w.h_ctl->flow_err)
-      HJ_DIE("dataflow sweep: a dependency wait timed out");
     const unsigned long long cvg = w.h_ctl->cvg;
     const unsigned skipped = w.h_ctl->skipped;
     const unsigned nsplit = w.h_ctl->split;
@@ -1226,7 +1017,6 @@ static void level2_noalloc(unsigned routine, unsigned nrow, unsigned ncol, unsig
   *glbSwp = blk_swp;
   launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
   HIP_CALL(hipStreamSynchronize(s));
-  g_last_flow = 0;
   g_last_stage = 0;
 }
 
@@ -1382,15 +1172,6 @@ extern "C" double hj_kernel_ms(int kind, unsigned long long *launches)
 extern "C" unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs) { return gram_nchunk(nrow, npairs); }
 
 extern "C" void hj_set_step_limit(long steps) { g_step_limit = steps; }
-
-extern "C" int hj_set_dataflow(int on)
-{
-  const int prev = g_flow;
-  g_flow = on ? 1 : 0;
-  return prev;
-}
-
-extern "C" int hj_last_dataflow(void) { return g_last_flow; }
 
 extern "C" int hj_set_staged(int mode)
 {

@@ -330,7 +330,6 @@ struct PairLds {
   float bnorm[2];
   int chol_ok;
   int go;     // the visit still runs inner sweeps
-  int ready;  // dataflow sweep: the visit's inputs are complete
 };
 
 // One pair visit by the 128 threads tid = 0..127 of a workgroup that may
@@ -793,8 +792,8 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const unsigned A0 = pairs[2u * Q.p0], B0 = pairs[2u * Q.p0 + 1u];
   const unsigned A1 = pairs[2u * Q.p1], B1 = pairs[2u * Q.p1 + 1u];
-  // the flags come from pair visits of the same launch in the dataflow
-  // sweep: atomic loads, never through the scalar cache
+  // flags written by another launch's pair visits: vector loads, never
+  // through the scalar cache
   const bool u0 = ld_relaxed(flags + Q.p0) != 0u, u1 = ld_relaxed(flags + Q.p1) != 0u;
   uint8_t pn0 = plan_next ? plan_next[Q.n0] : PLAN_FP64;
   uint8_t pn1 = plan_next ? plan_next[Q.n1] : PLAN_FP64;
@@ -912,170 +911,6 @@ void k_upgram(double *__restrict__ G, unsigned ldG, const uint16_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Dataflow sweep: all steps of a sweep in one persistent launch.  The
-// launch's work is a list of items in an order where every item's inputs
-// come from earlier items:
-//   UPGRAM(s, quad, chunk): the fused transition s -> s+1 of one closed quad
-//     over one row chunk (upgram_item), after the pair visits of step s of
-//     the quad's two pairs;
-//   PAIRS(s, up to 2 pairs): 128 threads per pair, the visit of k_pair
-//     (pair_visit: sum of the Gram partials, Cholesky, level-0 sweeps, W),
-//     after every chunk of the quad of step s-1 whose next pairs include it.
-// A workgroup takes the next item by an atomic ticket, waits (one lane,
-// bounded in time) until the counters of the item's inputs are complete,
-// runs it, and counts it done.  An item waits only on items of smaller
-// tickets, each already held by a running workgroup, so the waits end.
-// W, the update flags and the Gram partials live in rings of R steps; an
-// item that overwrites a ring slot first waits until the items of the step
-// that last used the slot are done.  The results are bit for bit those of
-// the step-by-step launches: every item runs the arithmetic of k_upgram /
-// k_pair on the same data.
-
-// the waits give up after this many 100 MHz clock ticks (2 s) and raise the
-// run's error word; every later wait then fails at once
-constexpr unsigned long long FLOW_TIMEOUT = 200000000ull;
-
-// the wave's later loads see what the producers released before the counts
-// this wave (or its workgroup, through a barrier) has observed
-__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-
-// one lane: wait until *p >= v (relaxed polling, one acquire fence at the
-// end); false when the run has failed
-__device__ bool flow_wait(const unsigned *p, unsigned v, unsigned *err)
-{
-  if (ld_relaxed(p) < v) {
-    const unsigned long long t0 = wall_clock64();
-    for (;;) {
-      __builtin_amdgcn_s_sleep(2);
-      if (ld_relaxed(p) >= v)
-        break;
-      if (ld_relaxed(err))
-        return false;
-      if (wall_clock64() - t0 > FLOW_TIMEOUT) {
-        atomicOr(err, 1u);
-        return false;
-      }
-    }
-  }
-  acquire_agent();
-  return true;
-}
-
-// pair_visit as a call of its own in the dataflow kernel: its register
-// allocation stays apart from the fused items'
-template <bool SORT>
-__device__ __noinline__ void flow_pair_visit(unsigned tid, bool present, unsigned pi, PairLds &L,
-                                             const int *go_other, const uint16_t *pairs,
-                                             const double *part, unsigned nchunk, double *Wout,
-                                             unsigned *flags, RunCtl *ctl, VisitRec *rec,
-                                             uint8_t plan, const uint8_t *S0, unsigned s0steps,
-                                             unsigned nswp, int definite, unsigned nplus)
-{
-  pair_visit<SORT>(tid, present, pi, L, go_other, pairs, part, nchunk, Wout, flags, ctl, rec, plan,
-                   S0, s0steps, nswp, definite, nplus);
-}
-
-// LDS of a flow workgroup: the fused item's, or two pair visits'
-union FlowLds {
-  UpgLds upg;
-  PairLds pair[2];
-};
-
-template <bool SORT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
-void k_flow(FlowArgs A)
-{
-  __shared__ FlowLds S;
-  __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
-  __shared__ unsigned s_item, s_ok;
-  const unsigned tid = threadIdx.x;
-  unsigned *const err = &A.ctl->flow_err;
-  if (tid < STRAT0_TAB_BYTES / 16u)
-    reinterpret_cast<uint4 *>(S0)[tid] = reinterpret_cast<const uint4 *>(A.strat0)[tid];
-  for (;;) {
-    if (tid == 0u)
-      s_item = atomicAdd(A.cnt, 1u);
-    __syncthreads();
-    const unsigned t = s_item;
-    if (t >= A.nitems)
-      break;
-    const FlowItem it = A.items[t];
-    const unsigned st = it.step;
-    const unsigned ring = st % A.R;
-    if (A.trace && tid == 0u)
-      A.trace[3u * (size_t)t] = wall_clock64();
-    if (it.kind == FLOW_UPGRAM) {
-      const Quad Q = A.quads[(size_t)st * A.nq + it.a[0]];
-      const unsigned *const pd = A.pdone + (size_t)st * A.np;
-      if (tid == 0u) {
-        // W and flags of both pairs; the partial slots of step st+1 free
-        bool ok = flow_wait(pd + Q.p0, 1u, err) && flow_wait(pd + Q.p1, 1u, err);
-        if (ok && st + 1u >= A.R)
-          ok = flow_wait(A.spg + (st + 1u - A.R), A.np, err);
-        s_ok = ok ? 1u : 0u;
-        if (A.trace)
-          A.trace[3u * (size_t)t + 1u] = wall_clock64();
-      }
-      __syncthreads();
-      if (s_ok) {
-        acquire_agent();
-        const unsigned nr = (st + 1u) % A.R;
-        upgram_item(S.upg, A.G, A.ldG, A.s1 + (size_t)st * A.np * 2u,
-                    A.W + (size_t)ring * A.np * 1024u, A.flags + (size_t)ring * A.np, Q, it.a[1],
-                    A.nchunk, A.CH, A.part + (size_t)nr * A.np * A.nchunk * 768u,
-                    A.plan ? A.plan + (size_t)(st + 1u) * A.np : nullptr, A.nw);
-        __threadfence();
-        __syncthreads();
-        if (tid == 0u) {
-          atomicAdd(A.qdone + (size_t)st * A.nq + it.a[0], 1u);
-          atomicAdd(A.supg + st, 1u);
-        }
-      }
-    }
-    else {
-      // two pair visits, one per half of the workgroup
-      const unsigned half = tid >> 7, ht = tid & 127u;
-      const unsigned pi = it.a[half];
-      if (ht == 0u) {
-        bool ok = pi != FLOW_NONE;
-        // the partials of the pair; the W / flag slots of step st free
-        if (ok && st)
-          ok = flow_wait(A.qdone + (size_t)(st - 1u) * A.nq + A.srcq[(size_t)st * A.np + pi],
-                         A.nchunk, err);
-        if (ok && st >= A.R)
-          ok = flow_wait(A.supg + (st - A.R), A.nq * A.nchunk, err);
-        S.pair[half].ready = ok ? 1 : 0;
-        if (A.trace && tid == 0u)
-          A.trace[3u * (size_t)t + 1u] = wall_clock64();
-      }
-      __syncthreads();
-      const bool present = S.pair[half].ready != 0;
-      __syncthreads();
-      acquire_agent();
-      flow_pair_visit<SORT>(ht, present, pi, S.pair[half], &S.pair[half ^ 1u].go,
-                       A.s1 + (size_t)st * A.np * 2u,
-                       A.part + (size_t)ring * A.np * A.nchunk * 768u, A.nchunk,
-                       A.W + (size_t)ring * A.np * 1024u, A.flags + (size_t)ring * A.np, A.ctl,
-                       A.rec + (size_t)st * A.np,
-                       visit_plan((present && A.plan) ? A.plan[(size_t)st * A.np + pi] : PLAN_FP64,
-                                  st == 0u),
-                       S0,
-                       A.s0steps, A.nswp, A.definite, A.nplus);
-      __threadfence();
-      __syncthreads();
-      if (present && ht == 0u) {
-        __hip_atomic_store(A.pdone + (size_t)st * A.np + pi, 1u, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        atomicAdd(A.spg + st, 1u);
-      }
-    }
-    __syncthreads();
-    if (A.trace && tid == 0u)
-      A.trace[3u * (size_t)t + 2u] = wall_clock64();
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Staged sweep: the fused transition of half the quads of step s runs in the
 // same launch as the visits of the pairs of another step whose inputs the
 // previous launch completed, so the visits' latency hides under the fused
@@ -1084,11 +919,17 @@ void k_flow(FlowArgs A)
 // the V workgroups (one (pair, row chunk) update of V per wave).  The
 // launch boundaries order everything else; the arithmetic is that of
 // k_pair / k_upgram / k_update.
+// LDS of a staged workgroup: a fused item's, or two pair visits'
+union StageLds {
+  UpgLds upg;
+  PairLds pair[2];
+};
+
 template <bool SORT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVES)))
 void k_stage(StageArgs A)
 {
-  __shared__ FlowLds S;
+  __shared__ StageLds S;
   __shared__ __attribute__((aligned(16))) uint8_t S0[STRAT0_TAB_BYTES];
   const unsigned tid = threadIdx.x, b = blockIdx.x;
   if (b < A.npw) {
@@ -1250,15 +1091,6 @@ void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, co
                      h2);
 }
 
-// counters of a flow launch to zero (a kernel, so the sweep graph holds no
-// memset node)
-__global__ void k_flow_reset(unsigned *cnt, size_t n)
-{
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x)
-    cnt[e] = 0u;
-}
-
 __global__ __launch_bounds__(256) void k_initD(double *__restrict__ G, double *__restrict__ D,
                                                unsigned ifc, unsigned nRow, unsigned nRank,
                                                unsigned nPlus, unsigned ldG)
@@ -1362,14 +1194,6 @@ void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs
                      quads, nchunk, CH, part, plan_next);
 }
 
-void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks)
-{
-  if (sort)
-    hipLaunchKernelGGL(k_flow<true>, dim3(nblocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_flow<false>, dim3(nblocks), dim3(256), 0, s, a);
-}
-
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
 {
   const unsigned nb = a.npw + a.niw + a.nvw;
@@ -1379,26 +1203,6 @@ void launch_stage(hipStream_t s, bool sort, const StageArgs &a)
     hipLaunchKernelGGL(k_stage<true>, dim3(nb), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(k_stage<false>, dim3(nb), dim3(256), 0, s, a);
-}
-
-unsigned flow_blocks_per_cu()
-{
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_flow<false>, 256, 0) != hipSuccess || n < 1)
-    n = 1;
-  int m = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_flow<true>, 256, 0) == hipSuccess && m >= 1 &&
-      m < n)
-    n = m;
-  return (unsigned)n;
-}
-
-void launch_flow_reset(hipStream_t s, unsigned *cnt, size_t n)
-{
-  size_t blocks = (n + 255u) / 256u;
-  if (blocks > 1024u) blocks = 1024u;
-  if (!blocks) blocks = 1u;
-  hipLaunchKernelGGL(k_flow_reset, dim3((unsigned)blocks), dim3(256), 0, s, cnt, n);
 }
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,

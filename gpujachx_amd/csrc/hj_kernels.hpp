@@ -58,7 +58,6 @@ struct RunCtl {
   float min_skip_frac;          // ... and only if the model proposes at least this fraction of skips
   float max_rot;                // most big rotations a sweep can have (visits x 496)
   unsigned proposed;            // skips the model proposed for the running sweep
-  unsigned flow_err;            // a wait of the dataflow sweep timed out (never cleared)
 };
 
 // waves sharing one Gram chunk of CH rows: each takes CH/nw rows, a multiple
@@ -88,46 +87,6 @@ bool closed_quads(const uint16_t *tab, unsigned steps, unsigned np, std::vector<
 void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
                    const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
                    unsigned CH, double *part, const uint8_t *plan_next);
-// Items of the dataflow sweep (k_flow): a fused transition of one closed
-// quad over one row chunk, or the visits of up to four pairs of one step
-// (one wave each).
-constexpr uint16_t FLOW_PAIRS = 0u;
-constexpr uint16_t FLOW_UPGRAM = 1u;
-constexpr uint16_t FLOW_NONE = 0xFFFFu;
-struct FlowItem {
-  uint16_t kind, step;
-  uint16_t a[4];  // FLOW_UPGRAM: quad, chunk; FLOW_PAIRS: pair slots (FLOW_NONE: unused wave)
-};
-
-// Everything one dataflow sweep reads and writes.  Rings of R steps: step s
-// uses slot s % R of W, flags and the Gram partials (the partials of step s
-// are those its pairs read).
-struct FlowArgs {
-  double *G;
-  unsigned ldG;
-  unsigned np, nq, nchunk, CH, nw, R;
-  unsigned nitems;
-  const uint16_t *s1;     // [steps][np][2] level-1 pairs
-  const Quad *quads;      // [steps-1][nq] closed quads of each transition
-  const uint16_t *srcq;   // [steps][np] quad of step s-1 whose next pairs hold pair p of step s
-  const FlowItem *items;  // [nitems] in dependency order
-  double *part;           // [R][np][nchunk][768]
-  double *W;              // [R][np][1024]
-  unsigned *flags;        // [R][np]
-  RunCtl *ctl;
-  VisitRec *rec;          // [steps][np]
-  const uint8_t *plan;    // [steps][np] visit plans, or null
-  const uint8_t *strat0;
-  unsigned s0steps, nswp;
-  int definite;
-  unsigned nplus;
-  unsigned *cnt;          // [0] ticket
-  unsigned *qdone;        // [steps][nq] chunks done per quad
-  unsigned *pdone;        // [steps][np] pair visit done
-  unsigned *supg;         // [steps] fused items done
-  unsigned *spg;          // [steps] pair visits done
-  unsigned long long *trace;  // [nitems][3] clock at ticket, inputs ready, done; or null
-};
 
 // One launch of a staged sweep: the visits of a list of pairs of one step
 // (two per workgroup) beside the fused transition of a range of closed quads
@@ -187,9 +146,6 @@ void launch_pair_qr(hipStream_t s, bool sort, bool drmac, const double *G, unsig
                     RunCtl *ctl, const uint8_t *strat0, unsigned s0steps, unsigned nswp,
                     int definite, unsigned nplus);
 
-void launch_flow(hipStream_t s, bool sort, const FlowArgs &a, unsigned nblocks);
-unsigned flow_blocks_per_cu();
-void launch_flow_reset(hipStream_t s, unsigned *cnt, size_t n);
 
 void launch_gram_split(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
                        unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);

@@ -256,12 +256,12 @@ void hj_last_sweeps(unsigned *nswp, unsigned *rot_s, unsigned *rot_b,
 
 /* Kernel timing of the last call (HIP events on the work stream).  With
  * hj_set_profile(1) (graphs are not used while profiling), per launch:
- * kind 0 = Gram, 1 = pair, 2 = update, 3 = the dataflow sweep kernel,
+ * kind 0 = Gram, 1 = pair, 2 = update, 3 = unused,
  * 4 = the launches of a staged sweep after step 0's visits (k_stage, and
  * the last step's update).  Without profiling, kind 5 = the graph-launched
  * sweeps: their GPU time (events around each graph launch) and the
  * launches of each sweep's dominant kind they hold (staged launches, else
- * one update-class launch per step, else one dataflow launch).  Returns
+ * one update-class launch per step).  Returns
  * total ms and launch count. */
 void hj_set_profile(int on);
 double hj_kernel_ms(int kind, unsigned long long *launches);
@@ -278,19 +278,6 @@ unsigned hj_set_max_sweeps(unsigned n);
 /* Test limit: stop after `steps` level-1 steps in total (0 = no limit). */
 void hj_set_step_limit(long steps);
 
-/* Dataflow sweeps (no counterpart in the reference, whose sweep is one
- * kernel launch per step, 1/HypJacL2.cu:131-148): every step of a sweep in
- * one persistent launch whose pair visits run while other pairs' block
- * updates stream through HBM, the steps ordered by device-side counters
- * instead of launch boundaries.  The results are bit for bit those of the
- * step-by-step launches.  Used when on, the strategy pairs every step
- * transition into closed quads (cycwor, rowcyc, colcyc, cycloc), V is not
- * accumulated and no step limit is set; otherwise the step-by-step path
- * runs.  Returns the previous setting.  hj_last_dataflow() tells whether
- * the last hypJacL2 / hypJacL2_dev call used them. */
-int hj_set_dataflow(int on);
-int hj_last_dataflow(void);
-
 /* Staged sweeps (no counterpart in the reference; same role as its
  * per-step launches, 1/HypJacL2.cu:131-148): each step transition runs as
  * two launches, each the fused block update + next Gram of half the step's
@@ -301,8 +288,7 @@ int hj_last_dataflow(void);
  * block updates read and write at least 4.8e8 bytes (512 (nrow + nV)
  * ncol/32; below that the visits' latency is longer than half a step's
  * updates and step-by-step launches are faster); 2: wherever they apply.
- * They apply when the dataflow sweep is off, every step transition forms
- * closed quads, ncol >= 128, no step limit is set and the factorization is
+ * They apply when every step transition forms closed quads, ncol >= 128, no step limit is set and the factorization is
  * Cholesky.  Returns the previous mode; hj_last_staged() tells whether the
  * last hypJacL2 / hypJacL2_dev call used them. */
 int hj_set_staged(int mode);
