@@ -54,10 +54,11 @@ __device__ __forceinline__ d4 sel4(unsigned i, d4 a, d4 b, d4 c, d4 d)
 #define UG_STORE(v, p) (*(p) = (v))
 #endif
 
-// Operands of one 16-row slab of a pair: for an updated pair the A operands
-// of [G_a G_b] W (lane l: row l%16 of pair column 4u + l/16), else the
-// unchanged slab in the Gram layout (x[r], x[4 + r]: row 4r + l/16 of
-// column l%16 of blocks a, b).
+// One 16-row slab of a pair, in the update's operand layout (lane l: row
+// l%16 of pair column 4u + l/16, x[u], u = 0..7; 128 contiguous bytes of a
+// column per instruction).  A pair without an update is read the same way
+// and turned into the Gram layout in LDS (slab_compute); la / lb false: no
+// next pair's Gram wants block a / b, nothing to read.
 __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned ldG, size_t row,
                                           unsigned a, unsigned b, bool upd, unsigned l, double *x,
                                           bool la = true, bool lb = true)
@@ -71,13 +72,28 @@ __device__ __forceinline__ void slab_load(const double *__restrict__ G, unsigned
     }
   }
   else {
-    // la / lb false: no next pair's Gram wants block a / b, nothing to read
-    const double *const ia = G + (size_t)(16u * a + (l & 15u)) * ldG + row + (l >> 4);
-    const double *const ib = G + (size_t)(16u * b + (l & 15u)) * ldG + row + (l >> 4);
+    // one branch per block (uniform), the loads of a block back to back
+    const double *const ia = G + (size_t)(16u * a + (l >> 4)) * ldG + row + (l & 15u);
+    const double *const ib = G + (size_t)(16u * b + (l >> 4)) * ldG + row + (l & 15u);
+    if (la) {
 #pragma unroll
-    for (unsigned r = 0u; r < 4u; ++r) {
-      x[r] = la ? UG_LOAD(ia + 4u * r) : 0.0;
-      x[4u + r] = lb ? UG_LOAD(ib + 4u * r) : 0.0;
+      for (unsigned u = 0u; u < 4u; ++u)
+        x[u] = UG_LOAD(ia + (size_t)(4u * u) * ldG);
+    }
+    else {
+#pragma unroll
+      for (unsigned u = 0u; u < 4u; ++u)
+        x[u] = 0.0;
+    }
+    if (lb) {
+#pragma unroll
+      for (unsigned u = 0u; u < 4u; ++u)
+        x[4u + u] = UG_LOAD(ib + (size_t)(4u * u) * ldG);
+    }
+    else {
+#pragma unroll
+      for (unsigned u = 0u; u < 4u; ++u)
+        x[4u + u] = 0.0;
     }
   }
 }
@@ -139,10 +155,27 @@ __device__ __forceinline__ void slab_compute(double *__restrict__ G, unsigned ld
     t1 = acc1;
   }
   else {
+    // no update: the slab to the Gram layout (lane l, register r: row
+    // 4r + l/16 of column l%16), one block at a time through T
 #pragma unroll
-    for (unsigned r = 0u; r < 4u; ++r) {
-      t0[r] = x[r];
-      t1[r] = x[4u + r];
+    for (unsigned h = 0u; h < 2u; ++h) {
+#pragma unroll
+      for (unsigned u = 0u; u < 4u; ++u)
+        T[(4u * u + (l >> 4)) * LDT + (l & 15u)] = x[4u * h + u];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      d4 t;
+#pragma unroll
+      for (unsigned r = 0u; r < 4u; ++r)
+        t[r] = T[(l & 15u) * LDT + 4u * r + (l >> 4)];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (h)
+        t1 = t;
+      else
+        t0 = t;
     }
   }
 }
