@@ -177,6 +177,7 @@ struct Work {
   uint16_t *d_plist = nullptr;
   uint16_t *d_plist_run = nullptr;  // d_plist reordered per sweep (learned skip)
   uint16_t *d_qord = nullptr;       // per transition, the quads in item order (learned skip)
+  unsigned *d_qact = nullptr;       // per transition and half, its quads with any work (learned skip)
   std::vector<char> cross;
 
   size_t keys() const { return (size_t)np * steps; }
@@ -283,6 +284,7 @@ struct Work {
     HIP_CALL(hipMemcpy(d_plist, pl.data(), sizeof(uint16_t) * pl.size(), hipMemcpyHostToDevice));
     HIP_CALL(hipMalloc(&d_plist_run, sizeof(uint16_t) * pl.size()));
     HIP_CALL(hipMalloc(&d_qord, sizeof(uint16_t) * (size_t)(steps - 1u) * (np / 2u)));
+    HIP_CALL(hipMalloc(&d_qact, sizeof(unsigned) * 2u * (size_t)(steps - 1u)));
     stage_ok = true;
     return true;
   }
@@ -303,6 +305,7 @@ struct Work {
     if (d_plist) HIP_CALL(hipFree(d_plist));
     if (d_plist_run) HIP_CALL(hipFree(d_plist_run));
     if (d_qord) HIP_CALL(hipFree(d_qord));
+    if (d_qact) HIP_CALL(hipFree(d_qact));
   }
 };
 
@@ -421,9 +424,27 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
 // workgroups and the visits' then all start at once, one round of at most
 // two per CU: plain N = 8192 4.99 -> 4.88 s; filling every free slot
 // instead, some workgroups taking two items, is slower: 140 against 124 us
-// per launch), one with the learned skip (skipped items make a static
-// pairing uneven; the dispatcher balances single items: 4.19 s against
-// 4.45 s with two).
+// per launch).  With the learned skip the launch holds one workgroup per
+// item, and the items with work, ordered first on the device, are taken one
+// per workgroup while they fit the slots beside the visits, else two per
+// workgroup (a static pairing of partly skipped items in index order is
+// uneven: 4.45 s against 4.19 s with one each; one each where every item
+// runs leaves a second round: 152 against 125 us per launch).
+static unsigned cu_count()
+{
+  static int cached_dev = -1;
+  static unsigned cached = 0u;
+  int dev = 0;
+  HIP_CALL(hipGetDevice(&dev));
+  if (dev != cached_dev) {
+    int n = 0;
+    HIP_CALL(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cached = n > 0 ? (unsigned)n : 1u;
+    cached_dev = dev;
+  }
+  return cached;
+}
+
 static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsigned nswp,
                                    int definite, unsigned nplus, double *dG, unsigned lddG,
                                    unsigned nrow, double *dV, unsigned lddV, unsigned ncol,
@@ -431,6 +452,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
 {
   const unsigned np = w.np, nq = np / 2u, h = nq / 2u, S = w.steps;
   const uint8_t *plan = planned ? w.d_skip : nullptr;
+  const unsigned ncu = cu_count();
   unsigned nev = 0u;
   auto mark = [&]() {
     if (ev) HIP_CALL(hipEventRecord(ev[nev], s));
@@ -483,6 +505,11 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
     x.qord = skipping ? w.d_qord + (size_t)st * nq + q0 : nullptr;
     x.nquads = n;
     x.niw = skipping ? n * w.nchunk : (n * w.nchunk + 1u) / 2u;
+    // with the learned skip the items with work come first and the item
+    // workgroups take one each while they fit the slots beside the visits
+    // (two resident workgroups per CU), else two each
+    x.qact = skipping ? w.d_qact + 2u * st + (q0 ? 1u : 0u) : nullptr;
+    x.islots = (2u * ncu > x.npw) ? 2u * ncu - x.npw : 1u;
     x.upart = w.part((st + 1u) & 1u);
     x.uplan = plan ? plan + (size_t)(st + 1u) * np : nullptr;
   };
@@ -492,7 +519,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   mark();
   if (skipping) {
     launch_order_visits(s, w.d_plist, w.d_plist_run, w.d_skip, np, S, 2u * h);
-    launch_order_quads(s, w.d_quads, w.d_qord, w.d_skip, np, S, h);
+    launch_order_quads(s, w.d_quads, w.d_qord, w.d_qact, w.d_skip, np, S, h);
   }
   launch_gram(s, dG, lddG, w.d_s1, np, w.nchunk, w.CH, w.part(0u), plan);
   if (split)

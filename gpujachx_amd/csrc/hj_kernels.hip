@@ -958,10 +958,17 @@ void k_stage(StageArgs A)
     }
   }
   else {
-    // item workgroup j takes items j, j + niw, j + 2 niw, ...
-    const unsigned nitems = A.nquads * A.nchunk;
-    const unsigned niw = A.niw;
-    for (unsigned i = b - A.npw; i < nitems; i += niw) {
+    // item workgroup j takes items j, j + stride, ...: stride niw, or with
+    // the learned skip (items with work first) one item each while they fit
+    // the slots beside the visits, else two each
+    unsigned nitems = A.nquads * A.nchunk;
+    unsigned stride = A.niw;
+    if (A.qact) {
+      nitems = min(nitems, ld_relaxed(A.qact) * A.nchunk);
+      stride = (nitems <= A.islots) ? nitems : (nitems + 1u) / 2u;
+    }
+    const unsigned j = b - A.npw;
+    for (unsigned i = j; i < nitems && j < stride; i += stride) {
       const unsigned qi = i / A.nchunk, c = i - qi * A.nchunk;
       // a copy in registers: through a reference into global memory its
       // byte fields would be reloaded after every store of the item
@@ -1030,6 +1037,7 @@ __global__ __launch_bounds__(256) void k_order_visits(const uint16_t *__restrict
 // half); a half holds at most 64 ... nq/2 quads, looped in chunks of 64.
 __global__ __launch_bounds__(64) void k_order_quads(const Quad *__restrict__ quads,
                                                     uint16_t *__restrict__ out,
+                                                    unsigned *__restrict__ act,
                                                     const uint8_t *__restrict__ plan,
                                                     unsigned np, unsigned h)
 {
@@ -1059,6 +1067,9 @@ __global__ __launch_bounds__(64) void k_order_quads(const Quad *__restrict__ qua
       start[k] = acc;
       acc += cnt[k];
     }
+    // quads with any work (weight < 4), a vector store
+    if (l == 0u)
+      act[blockIdx.x] = n - cnt[4];
   }
   const unsigned long long below = (l == 0u) ? 0ull : (~0ull >> (64u - l));
   for (unsigned c0 = 0u; c0 < n; c0 += 64u) {
@@ -1078,10 +1089,11 @@ __global__ __launch_bounds__(64) void k_order_quads(const Quad *__restrict__ qua
   }
 }
 
-void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, const uint8_t *plan,
-                        unsigned np, unsigned steps, unsigned h)
+void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, unsigned *act,
+                        const uint8_t *plan, unsigned np, unsigned steps, unsigned h)
 {
-  hipLaunchKernelGGL(k_order_quads, dim3(2u * (steps - 1u)), dim3(64), 0, s, quads, out, plan, np, h);
+  hipLaunchKernelGGL(k_order_quads, dim3(2u * (steps - 1u)), dim3(64), 0, s, quads, out, act, plan,
+                     np, h);
 }
 
 void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,

@@ -112,6 +112,8 @@ struct StageArgs {
   const unsigned *uflags;  // update flags of that step
   const Quad *quads;       // first quad of the range (with qord: of the transition)
   const uint16_t *qord;    // the range's quads in item order, or null (in index order)
+  const unsigned *qact;    // with qord: how many of them (first in that order) have work, or null
+  unsigned islots;         // with qact: item workgroups that fit beside the visits at once
   unsigned nquads;
   unsigned nchunk, CH, nw;
   unsigned niw;            // item workgroups (1..nquads*nchunk; 0 without items)
@@ -133,8 +135,8 @@ struct StageArgs {
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a);
 // quads of a staged sweep's transitions, heaviest items first (learned skip)
-void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, const uint8_t *plan,
-                        unsigned np, unsigned steps, unsigned h);
+void launch_order_quads(hipStream_t s, const Quad *quads, uint16_t *out, unsigned *act,
+                        const uint8_t *plan, unsigned np, unsigned steps, unsigned h);
 // visit lists of a staged sweep, running pairs first (learned skip)
 void launch_order_visits(hipStream_t s, const uint16_t *plist, uint16_t *out, const uint8_t *plan,
                          unsigned np, unsigned steps, unsigned h2);
