@@ -158,11 +158,12 @@ __device__ void wave_update(double *__restrict__ M, unsigned ld, unsigned nrow, 
 
 // The visit of one block pair by one wave on the Gram in R (BR layout, the
 // lower triangle): Cholesky, level-0 sweep(s), W in W.  Returns the rotation
-// counts of the pair (s, b) in every lane.
+// counts of the pair (s, b) in every lane, and whether W differs from the
+// identity (a rotation or a swap: the update is left out otherwise).
 template <bool SORT>
 __device__ __noinline__ void wave_visit(const BProb &P, unsigned iBlk, unsigned jBlk, double *R, double *W,
                            const uint8_t *S0, unsigned s0steps, unsigned nswp, int definite,
-                           unsigned l, unsigned &out_s, unsigned &out_b)
+                           unsigned l, unsigned &out_s, unsigned &out_b, bool &out_upd)
 {
   // local J structure, share/device_code_nosort_accumV.hpp:53-76
   bool hyp = false;
@@ -195,10 +196,13 @@ __device__ __noinline__ void wave_visit(const BProb &P, unsigned iBlk, unsigned 
   // level-0 sweeps (slot y = l/4 of the 16 pivot pairs, lane l%4 of the
   // slot holding rows l%4 + 4k)
   unsigned blk_s = 0u, blk_b = 0u;
+  bool swapped = false;
   double cos_unused = 0.0;
-  wave_level0<SORT, false>(R, W, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, cos_unused);
+  wave_level0<SORT, false>(R, W, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, swapped,
+                           cos_unused);
   out_s = blk_s;
   out_b = blk_b;
+  out_upd = (blk_s != 0u) || swapped;
 }
 
 // One block pair by one wave: Gram, visit, update (the unfused step).
@@ -209,8 +213,9 @@ __device__ void wave_pair(const BProb &P, unsigned iBlk, unsigned jBlk, double *
 {
   wave_gram(P.G, P.ldG, P.nrow, iBlk, jBlk, R, l);
   wave_lds_sync();
-  wave_visit<SORT>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, out_s, out_b);
-  if (SORT || out_s) {
+  bool upd = false;
+  wave_visit<SORT>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, out_s, out_b, upd);
+  if (upd) {
     wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l, BLDR);
     if (P.routine & HYPJAC_FULL_SVD)
       wave_update(P.V, P.ldV, P.ncol, iBlk, jBlk, W, l, BLDR);
@@ -368,13 +373,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BATCH_WAVES
             gram_to_R(sg + (size_t)pi * 768u, R, l);
           wave_lds_sync();
           unsigned s = 0u, b = 0u;
+          bool upd = false;
           if (sort)
-            wave_visit<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            wave_visit<true>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b, upd);
           else
-            wave_visit<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b);
+            wave_visit<false>(P, iBlk, jBlk, R, W, S0, s0steps, nswp, definite, l, s, b, upd);
           sw_s += s;
           sw_b += b;
-          const bool upd = sort || s;
           if (!fused) {
             if (upd) {
               wave_update(P.G, P.ldG, P.nrow, iBlk, jBlk, W, l, BLDR);

@@ -313,18 +313,20 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v)
 // touch disjoint columns, so one wave-level LDS sync per step suffices.
 // Returns the rotation counts of all its sweeps (the reference's counting:
 // a sweep without a rotation ends the loop and adds nothing) in every lane,
-// and with COS the largest |A_pq| / sqrt(A_pp A_qq) over the pivots of the
-// first sweep.
+// whether a pivot pair was swapped by the sort rule (W is the identity
+// exactly when nothing was rotated or swapped), and with COS the largest
+// |A_pq| / sqrt(A_pp A_qq) over the pivots of the first sweep.
 template <bool SORT, bool COS>
 __device__ __forceinline__ void wave_level0(double *R_, double *W_, const uint8_t *S0,
                                             unsigned s0steps, unsigned nswp, int def, bool hyp,
                                             unsigned npos, unsigned l, unsigned &out_s,
-                                            unsigned &out_b, double &out_cos)
+                                            unsigned &out_b, bool &out_swap, double &out_cos)
 {
   lds_f64 *const R = (lds_f64 *)R_;
   lds_f64 *const W = (lds_f64 *)W_;
   const unsigned y = l >> 2, r = l & 3u;
   unsigned blk_s = 0u, blk_b = 0u;
+  bool swapped = false;
   double my_cos = 0.0;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned my_s = 0u, my_b = 0u;
@@ -406,6 +408,7 @@ __device__ __forceinline__ void wave_level0(double *R_, double *W_, const uint8_
           L0A(W, r + 4u * k, p) = Vq[k];
           L0A(W, r + 4u * k, q) = Vp[k];
         }
+        swapped = true;
       }
       my_s += (unsigned)transf_s;
       my_b += (unsigned)transf_b;
@@ -422,6 +425,7 @@ __device__ __forceinline__ void wave_level0(double *R_, double *W_, const uint8_
   }
   out_s = blk_s;
   out_b = blk_b;
+  out_swap = __ballot(swapped) != 0ull;
   if (COS) {
 #pragma unroll
     for (int o = 32; o >= 4; o >>= 1)

@@ -330,6 +330,7 @@ struct PairLds {
   float bnorm[2];
   int chol_ok;
   int go;     // the visit still runs inner sweeps
+  int swp;    // a pivot pair was swapped by the sort rule
 };
 
 // One pair visit by the 128 threads tid = 0..127 of a workgroup that may
@@ -349,6 +350,8 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
 {
   double *const R = L.R;
   double *const W = L.W;
+  if (tid == 0u)
+    L.swp = 0;
   if (present && plan == PLAN_SKIP && tid == 0u) {
     // skipped visit: no update, the record keeps the last executed visit
     flags[pi] = 0u;
@@ -449,6 +452,7 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
   // disjoint columns; rotation counts are kept per slot and summed per sweep.
   const unsigned y = tid >> 3, r = tid & 7u;
   unsigned blk_s = 0u, blk_b = 0u;
+  bool swapped = false;
   double my_cos = 0.0;
   // |Apq| / (|p| |q|) of the previous step, divided during the next step's
   // loads (off the step's chain of dependent operations)
@@ -531,6 +535,7 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
           RA(W, r + 8u * i, p) = Vq[i];
           RA(W, r + 8u * i, q) = Vp[i];
         }
+        swapped = true;
       }
       my_s += (unsigned)transf_s;
       my_b += (unsigned)transf_b;
@@ -565,6 +570,8 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
 
   if (live && r == 0u)
     L.cmax[y] = (float)my_cos;
+  if (live && swapped)
+    L.swp = 1;
   __syncthreads();
   if (!live)
     return;
@@ -577,7 +584,9 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
       else
         atomicAdd(reinterpret_cast<unsigned *>(cvg), blk_s);
     }
-    flags[pi] = SORT ? 1u : (blk_s ? 1u : 0u);
+    // the block update is left out exactly when W is the identity (no
+    // rotation, no swap)
+    flags[pi] = (blk_s || L.swp) ? 1u : 0u;
     float cm = 0.0f;
     for (unsigned i = 0u; i < 16u; ++i)
       cm = fmaxf(cm, L.cmax[i]);

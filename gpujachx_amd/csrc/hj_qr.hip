@@ -540,7 +540,9 @@ __global__ __launch_bounds__(64) void k_pair_qr(const double *__restrict__ G, un
   }
   else {
     double unused_cos;
-    wave_level0<SORT, false>(R, P, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, unused_cos);
+    bool unused_swap;
+    wave_level0<SORT, false>(R, P, S0, s0steps, nswp, def, hyp, npos, l, blk_s, blk_b, unused_swap,
+                             unused_cos);
   }
   wave_lds_sync();
 

@@ -239,12 +239,17 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
 /* Inner sweep(s) on R (32x32) accumulating W; reference
  * share/device_code_nosort.hpp:4-222 (sort == 0) and
  * share/device_code_cdsort_KISS.hpp:4-260 + share/device_code_cdsort.hpp:25-167
- * (sort == 1).  hyp: the block pair mixes J signs (local npos). */
+ * (sort == 1).  hyp: the block pair mixes J signs (local npos).
+ * swapped_out (may be NULL): whether a pivot pair was swapped without a
+ * rotation (the sort rule); W stays the identity exactly when no pivot pair
+ * was rotated or swapped. */
 void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
                unsigned npos, const uint8_t *strat0, unsigned s0steps,
-               unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out)
+               unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out,
+               int *swapped_out)
 {
   unsigned blk_s = 0u, blk_b = 0u;
+  int swapped = 0;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned swp_s = 0u, swp_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
@@ -335,6 +340,7 @@ void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
               F32(R, x, p) = Gq[x]; F32(R, x, q) = Gp[x];
               F32(W, x, p) = Vq[x]; F32(W, x, q) = Vp[x];
             }
+            swapped = 1;
           }
         }
         swp_b += (unsigned)transf_b;
@@ -349,6 +355,8 @@ void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
   }
   *blk_s_out = blk_s;
   *blk_b_out = blk_b;
+  if (swapped_out)
+    *swapped_out = swapped;
 }
 
 /* Level-1 block-pair body: share/device_code_cdsort_accumV.hpp:35-94 and its
@@ -392,11 +400,16 @@ unsigned hjo_pair(const double *partials, unsigned nchunk, int sort,
     for (unsigned x = 0u; x < 32u; ++x)
       F32(W, x, y) = (x == y) ? 1.0 : 0.0;
   unsigned s = 0u, b = 0u;
-  hjo_l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b);
+  int swapped = 0;
+  hjo_l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b, &swapped);
   if (Rout)
     memcpy(Rout, R, sizeof(R));
   *big = b;
-  *upd = sort ? 1 : (s != 0u);
+  /* the block update is left out exactly when W is the identity: no pivot
+   * pair rotated or (sorted variants) swapped; multiplying by it would only
+   * turn negative zeros positive (this build's rule: the reference always
+   * multiplies in its sorted variants) */
+  *upd = (s != 0u) || swapped;
   return s;
 }
 

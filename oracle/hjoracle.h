@@ -82,17 +82,20 @@ void hjo_gram_partial(const double *G0, const double *G1, unsigned ld,
  * producing the 32x32 column-major transformation W.
  * kind: 0 = definite (definite = +1/-1), 1 = hyperbolic (npos local).
  * Returns the block's rotation count; *big = big rotations;
- * *upd = 1 if the block update must be applied. */
+ * *upd = 1 if the block update must be applied: unless W is the identity
+ * (no pivot pair rotated or swapped). */
 unsigned hjo_pair(const double *partials, unsigned nchunk, int sort,
                   int hyp, int definite, unsigned npos,
                   const uint8_t *strat0, unsigned s0steps, unsigned nswp,
                   double W[1024], unsigned *big, int *upd, double R[1024]);
 
 /* level-0 sweeps on R (32x32 upper triangular, column-major) accumulating
- * W (identity on entry); returns the rotation counts. */
+ * W (identity on entry); returns the rotation counts and (swapped_out, may
+ * be NULL) whether a pivot pair was swapped by the sort rule. */
 void hjo_l0(double *R, double *W, int sort, int hyp, int definite,
             unsigned npos, const uint8_t *strat0, unsigned s0steps,
-            unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out);
+            unsigned nswp, unsigned *blk_s_out, unsigned *blk_b_out,
+            int *swapped_out);
 
 /* Factorization of the block pairs: Cholesky of the Gram (the reference's
  * default build), QR by 32-row panels (USE_QR), QR with Drmac's sorted

@@ -451,7 +451,7 @@ unsigned hjo_pair_qr(const double *G0, const double *G1, unsigned ld, unsigned n
   if (drmac && sort && !hyp)
     hjo_l0_drmac(R, W, definite, strat0, s0steps, nswp, &s, &b);
   else
-    hjo_l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b);
+    hjo_l0(R, W, sort, hyp, definite, npos, strat0, s0steps, nswp, &s, &b, NULL);
   *big = b;
   *upd = sort ? 1 : (s != 0u);
   return s;

@@ -116,3 +116,26 @@ def test_strategies_full(snp):
     G = svd_input(256, 256, 10.0 ** (-3.0 * np.arange(256) / 255), seed=3)
     rg, ro = run_both(9, G, 256, snp1=snp, snp0=snp)
     assert_same(rg, ro, snp)
+
+
+@pytest.mark.parametrize("alg", [9, 1, 8])
+@pytest.mark.parametrize("descending", [True, False])
+def test_identity_updates_left_out(alg, descending):
+    # mutually orthogonal columns (disjoint rows): every Gram is diagonal, no
+    # pivot pair rotates; in the sorted variants (ALG 9, 1) unsorted norms
+    # make the sort rule swap columns (an update by a permutation), sorted
+    # ones leave W the identity and the update out.  GPU and oracle agree on
+    # every bit, and a run without swaps leaves every column in its place.
+    m, n = 256, 64
+    d = 10.0 ** -np.linspace(0.0, 6.0, n)
+    if not descending:
+        d = d[::-1].copy()
+    rows = np.random.default_rng(5).permutation(m)[:n]
+    G = np.zeros((m, n))
+    G[rows, np.arange(n)] = d
+    rg, ro = run_both(alg, G, n)
+    assert_same(rg, ro, "alg %d descending %s" % (alg, descending))
+    assert rg.rot_s == 0
+    # the output columns are normalised; without swaps each keeps its row
+    same_pattern = np.array_equal(rg.G != 0.0, G != 0.0)
+    assert same_pattern == (descending or not (alg & 1))

@@ -144,7 +144,34 @@ def test_pair_cholesky_and_transformation():
                 dif = np.diff(norms)
                 assert (np.all(dif <= norms[:-1] * 1e-12) if definite > 0
                         else np.all(dif >= -norms[1:] * 1e-12))
-            assert upd == 1 or (sort == 0 and s == 0)
+            # the update is left out exactly when W is the identity
+            assert upd == int(s > 0 or not np.array_equal(W, np.eye(32)))
+
+
+def test_pair_update_left_out_only_for_the_identity():
+    # a diagonal Gram: no pivot pair rotates; sorted norms (descending for
+    # J = I, ascending for J = -I) leave W the identity and the update out,
+    # the reverse orders make the sort rule swap: W a permutation, update on
+    s0 = g.strat_generate("cycwor", 32).astype(np.uint8)
+    d = 10.0 ** -np.arange(32, dtype=float) * 3.0
+    lanes = np.arange(64)
+    for definite, order in ((1, d), (1, d[::-1]), (-1, d[::-1]), (-1, d)):
+        gram = np.diag(order)
+        P = np.zeros(768)
+        for tile, (ri, cj) in enumerate(((0, 0), (16, 0), (16, 16))):
+            for r in range(4):
+                P[tile * 256 + lanes * 4 + r] = gram[ri + 4 * r + lanes // 16, cj + lanes % 16]
+        sorted_already = np.all(np.diff(order) < 0) if definite > 0 else np.all(np.diff(order) > 0)
+        for sort in (0, 1):
+            W, R, s, b, upd = o.pair(P, sort, 0, definite, 32 if definite > 0 else 0, s0, nswp=30)
+            assert s == 0 and b == 0
+            identity = np.array_equal(W, np.eye(32))
+            assert identity == (sort == 0 or sorted_already)
+            assert upd == int(not identity)
+            if not identity:
+                # a permutation: the swaps of the sort rule
+                assert np.array_equal(np.sort(W, axis=0)[-1], np.ones(32))
+                assert np.count_nonzero(W) == 32
 
 
 @pytest.mark.parametrize("alg", [9, 11, 1, 3])
