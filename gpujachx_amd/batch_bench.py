@@ -105,8 +105,9 @@ def roofline(g, probs, sweeps, strategy, alg, seconds):
             "peaks": "FP64 MFMA 78.6 TFLOP/s, HBM 8 TB/s (MI355X published)",
             "bound": "hbm",
             "note": "counter runs of the same workload (profiles/roofline_batch_4096.md): measured "
-                    "HBM traffic 0.96x the algorithmic bytes at 4.65 TB/s, FP64 MFMA busy 28 %: "
-                    "bound by HBM, the visits' latency taking the rest"}
+                    "HBM traffic 0.79x the algorithmic bytes (updates by an identity W are left "
+                    "out) at 4.89 TB/s, FP64 MFMA busy 30 %: bound by HBM, the visits' latency "
+                    "taking the rest"}
 
 
 def main():
