@@ -91,7 +91,10 @@ __device__ __forceinline__ double dev_rsqrt(double x)
 // 2x2 rotation of share/device_code_common_rotate.hpp:189-237 (USE_RSQRT),
 // trigonometric (trig) or hyperbolic, evaluated without divergent branches:
 // every path's operations are issued and the taken branch's results are
-// selected, so the values equal the branch formulas bit for bit.
+// selected, so the values equal the branch formulas bit for bit.  The square
+// root and the reciprocal square root are skipped by a wave-uniform branch
+// when no active lane of the wave takes a branch that uses them (a nearly
+// converged wave: every pivot pair's |zeta| at or above sqrt(2/eps), c = 1).
 //   trig: zeta = (Dq-Dp)/(2Apq); |zeta| < sqrt(eps/2): t = 1/(|zeta|+1),
 //         |zeta| >= sqrt(2/eps): t = 1/(2|zeta|), c = 1, else
 //         t = 1/(|zeta| + sqrt(zeta^2+1)); c = rsqrt(1+t^2); big = c < 1.
@@ -108,11 +111,15 @@ __device__ __forceinline__ int dev_rot(bool trig, double Apq, double Dp, double 
   const bool large = CT2 >= SQRT_2_HEPS;
   const bool small = trig && (CT2 < SQRT_HEPS);
   const bool fix = !trig && !large && !(CT2 > 1.0);
-  const double sq = pin(sqrt(__fma_rn(CT2, CT2, trig ? 1.0 : -1.0)));
+  double sq = 0.0;
+  if (__ballot(!large && !small && !fix) != 0ull)
+    sq = pin(sqrt(__fma_rn(CT2, CT2, trig ? 1.0 : -1.0)));
   const double cot = large ? 2.0 * CT2 : (small ? CT2 + 1.0 : CT2 + sq);
   const double rcot = pin(1.0 / cot);  // (a statement of its own: a pin inside ?: runs on one arm)
   const double tt = fix ? TANH1_FIX : copysign(rcot, ct2);
-  const double cr = pin(dev_rsqrt(__fma_rn(trig ? tt : -tt, tt, 1.0)));
+  double cr = 1.0;
+  if (__ballot(!large && !fix) != 0ull)
+    cr = pin(dev_rsqrt(__fma_rn(trig ? tt : -tt, tt, 1.0)));
   c = large ? 1.0 : (fix ? COSHI_FIX : cr);
   t = tt;
   return large ? 0 : ((small || fix) ? 1 : (trig ? (cr < 1.0) : (cr > 1.0)));
