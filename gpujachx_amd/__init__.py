@@ -124,6 +124,8 @@ def lib():
     L.hj_mlp_train.restype = i
     L.hj_set_skip_min_frac.argtypes = [ctypes.c_float]
     L.hj_set_skip_min_frac.restype = None
+    L.hj_set_skip_floor_mixed.argtypes = [ctypes.c_float]
+    L.hj_set_skip_floor_mixed.restype = None
     L.hj_set_skip_min_propose.argtypes = [ctypes.c_float]
     L.hj_set_skip_min_propose.restype = None
     L.hj_set_skip_min_visits.argtypes = [ctypes.c_uint]

@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=80)
     ap.add_argument("--model", default=None)
     ap.add_argument("--skip-min-frac", type=float, default=None)
+    ap.add_argument("--skip-floor-mixed", type=float, default=None,
+                    help="floor multiple for runs that mix J signs (default 3)")
     args = ap.parse_args()
     os.environ.setdefault("HJ_QUIET", "1")
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -127,6 +129,8 @@ def main():
     g.set_max_sweeps(args.max_sweeps)
     if args.skip_min_frac is not None:
         g.lib().hj_set_skip_min_frac(args.skip_min_frac)
+    if args.skip_floor_mixed is not None:
+        g.lib().hj_set_skip_floor_mixed(args.skip_floor_mixed)
     for name in args.configs:
         cfg = CONFIGS[name]
         Gt, s = make(torch, cfg, args.seed)

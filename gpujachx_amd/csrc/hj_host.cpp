@@ -46,6 +46,9 @@ static float g_split_tau = 1e-3f;  // split Gram while the pair's last max |cos|
 // skipping only while the previous sweep's big rotations are at least this
 // fraction of the most a sweep can have (the last sweeps run unskipped)
 static float g_skip_min_frac = 0.01f;
+// runs that mix J signs skip only above this multiple of the floor (their
+// skips cost more extra sweeps: N = 8192 and 16384, measured)
+static float g_skip_floor_mixed = 3.0f;
 // ... and only in sweeps where the model proposes at least this fraction of
 // all visits
 static float g_skip_min_propose = 0.1f;
@@ -617,7 +620,7 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     c.threshold = g_skip_threshold;
     c.max_skips = g_skip_max;
     c.tau = g_split_tau;
-    c.min_frac = g_skip_min_frac;
+    c.min_frac = g_skip_min_frac * ((definite == 0) ? g_skip_floor_mixed : 1.0f);
     c.min_skip_frac = g_skip_min_propose;
     c.max_rot = max_rot;
     HIP_CALL(hipMemcpyAsync(w.d_ctl, w.h_ctl, sizeof(RunCtl), hipMemcpyHostToDevice, s));
@@ -1132,6 +1135,9 @@ extern "C" void hj_last_skips(unsigned *skip, unsigned long long *skipped, unsig
 extern "C" void hj_set_split_tau(float tau) { g_split_tau = tau; }
 
 extern "C" void hj_set_skip_min_frac(float frac) { g_skip_min_frac = frac; }
+
+// The floor of runs that mix J signs, as a multiple of hj_set_skip_min_frac's
+extern "C" void hj_set_skip_floor_mixed(float factor) { g_skip_floor_mixed = factor; }
 
 extern "C" void hj_set_skip_min_propose(float frac) { g_skip_min_propose = frac; }
 
