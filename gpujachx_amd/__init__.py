@@ -132,6 +132,8 @@ def lib():
     L.hj_set_skip_min_visits.restype = None
     L.hj_set_split_tau.argtypes = [ctypes.c_float]
     L.hj_set_split_tau.restype = None
+    L.hj_set_fused_split.argtypes = [ctypes.c_int]
+    L.hj_set_fused_split.restype = ctypes.c_int
     L.hj_last_splits.argtypes = [pu, pull]
     L.hj_last_splits.restype = None
     L.hj_dbg_gram.argtypes = [i, u, pd, pd]

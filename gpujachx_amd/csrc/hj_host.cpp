@@ -43,6 +43,13 @@ static uint16_t g_model_bf16[MLP_NPARAM];
 static float g_skip_threshold = 0.05f;
 static unsigned g_skip_max = 2u;
 static float g_split_tau = 1e-3f;  // split Gram while the pair's last max |cos| > tau
+// the fused transitions form the split Gram of split-planned pairs (else
+// only the standalone Gram launches do, the fused ones FP64).  Off by
+// default: at N = 8192 (bench matrix class, two seeds) the learned+split
+// SVD took 0-5 % longer with it and the HSVD (NPLUS = N/2) 6-7 % (a sweep
+// more): the splits' VALU work and registers cost the fused item more than
+// the BF16 MFMAs save over the FP64 Gram's.
+static int g_fused_split = 0;
 // skipping only while the previous sweep's big rotations are at least this
 // fraction of the most a sweep can have (the last sweeps run unskipped)
 static float g_skip_min_frac = 0.01f;
@@ -387,17 +394,18 @@ static unsigned issue_sweep(hipStream_t s, const Work &w, bool sort, unsigned ns
         launch_gram_split(s, dG, lddG, pairs, w.np, w.nchunk, w.CH, w.d_part, plan);
     }
     if (ev) HIP_CALL(hipEventRecord(ev[1], s));
-    // a split-planned visit holds a split Gram only after a standalone Gram
-    // launch; after a fused transition its Gram is FP64
+    // a split-planned visit holds a split Gram after a standalone Gram
+    // launch, and after a fused transition that forms split Grams
     launch_pair(s, sort, pairs, w.np, w.d_part, w.nchunk, w.d_W, w.d_flags, w.d_ctl, w.d_rec + off,
-                plan, split && !have_gram, w.d_s0, w.s0steps, nswp, definite, nplus);
+                plan, split && (!have_gram || g_fused_split), w.d_s0, w.s0steps, nswp, definite,
+                nplus);
     if (ev) HIP_CALL(hipEventRecord(ev[2], s));
     const bool fused = (st + 1u < w.steps) && w.fuse[st] &&
                        !(steps_left && *steps_left <= 0);
     if (fused) {
       launch_upgram(s, dG, lddG, pairs, w.d_W, w.d_flags, w.d_quads + (size_t)st * (w.np / 2u),
                     w.np / 2u, w.nchunk, w.CH, w.d_part,
-                    planned ? w.d_skip + off + w.np : nullptr);
+                    planned ? w.d_skip + off + w.np : nullptr, split && g_fused_split);
       if (dV)
         launch_update(s, dG, lddG, 0u, w.chG, dV, lddV, ncol, w.chV, pairs, w.np, w.d_W, w.d_flags);
     }
@@ -473,6 +481,7 @@ static unsigned issue_sweep_staged(hipStream_t s, const Work &w, bool sort, unsi
   base.nswp = nswp;
   base.definite = definite;
   base.nplus = nplus;
+  base.fsplit = (split && g_fused_split) ? 1 : 0;
   auto visits = [&](StageArgs &x, unsigned st, const uint16_t *list, unsigned n) {
     x.npl = n;
     x.npw = (n + 1u) / 2u;
@@ -1133,6 +1142,12 @@ extern "C" void hj_last_skips(unsigned *skip, unsigned long long *skipped, unsig
 }
 
 extern "C" void hj_set_split_tau(float tau) { g_split_tau = tau; }
+extern "C" int hj_set_fused_split(int on)
+{
+  const int old = g_fused_split;
+  g_fused_split = on ? 1 : 0;
+  return old;
+}
 
 extern "C" void hj_set_skip_min_frac(float frac) { g_skip_min_frac = frac; }
 

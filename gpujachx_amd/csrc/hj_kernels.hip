@@ -609,8 +609,8 @@ __device__ __forceinline__ void pair_visit(unsigned tid, bool present, unsigned 
 }
 
 // The plan a visit runs: a visit planned split whose Gram partials come from
-// the FP64 Gram (a fused transition's, which forms the FP64 Gram for every
-// pair it serves) is an FP64 visit -- it is counted and checked as one.
+// the FP64 Gram (a fused transition's that forms FP64 Grams only) is an FP64
+// visit -- it is counted and checked as one.
 __device__ __forceinline__ uint8_t visit_plan(uint8_t plan, bool split_src)
 {
   return (plan == PLAN_SPLIT && !split_src) ? PLAN_FP64 : plan;
@@ -787,16 +787,17 @@ struct UpgLds {
 
 // One (quad, row chunk) of the fused transition by the workgroup's first nw
 // waves (the others only pass the barriers).  Every non-skipped next pair
-// gets the FP64 Gram, whatever its plan: riding on the update's HBM stream
-// the Gram adds no memory traffic, and the kernel is bound by HBM, not by
-// the FP64 MFMA units.
+// gets its Gram from the updated slabs in registers: the split Gram (the
+// slab arithmetic and partial layout of k_gram_split) when fsplit is set and
+// the pair is planned split, else the FP64 Gram.
 __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, unsigned ldG,
                                             const uint16_t *__restrict__ pairs,
                                             const double *__restrict__ Wbuf,
                                             const unsigned *__restrict__ flags, const Quad &Q,
                                             unsigned c, unsigned nchunk, unsigned CH,
                                             double *__restrict__ part,
-                                            const uint8_t *__restrict__ plan_next, unsigned nw)
+                                            const uint8_t *__restrict__ plan_next, unsigned nw,
+                                            bool fsplit)
 {
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const unsigned A0 = pairs[2u * Q.p0], B0 = pairs[2u * Q.p0 + 1u];
@@ -814,6 +815,7 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
   const bool nt1 = (g0 && (Q.x0 == 1u || Q.y0 == 1u)) || (g1 && (Q.x1 == 1u || Q.y1 == 1u));
   const bool nt2 = (g0 && (Q.x0 == 2u || Q.y0 == 2u)) || (g1 && (Q.x1 == 2u || Q.y1 == 2u));
   const bool nt3 = (g0 && (Q.x0 == 3u || Q.y0 == 3u)) || (g1 && (Q.x1 == 3u || Q.y1 == 3u));
+  const bool sp0 = fsplit && pn0 == PLAN_SPLIT, sp1 = fsplit && pn1 == PLAN_SPLIT;
   // W(k, j) of both pairs into LDS (row-major, padded)
   for (unsigned e = threadIdx.x; e < 2048u; e += blockDim.x) {
     const unsigned pp = e >> 10, k = (e >> 5) & 31u, j = e & 31u;
@@ -874,11 +876,17 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
 #endif
       if (pn0 != PLAN_SKIP) {
         const d4 X = sel4(Q.x0, t0, t1, t2, t3), Y = sel4(Q.y0, t0, t1, t2, t3);
-        gram_fp64_slab(X, Y, g00, g10, g11);
+        if (sp0)
+          gram_split_slab(X, Y, l, g00, g10, g11);
+        else
+          gram_fp64_slab(X, Y, g00, g10, g11);
       }
       if (pn1 != PLAN_SKIP) {
         const d4 X = sel4(Q.x1, t0, t1, t2, t3), Y = sel4(Q.y1, t0, t1, t2, t3);
-        gram_fp64_slab(X, Y, h00, h10, h11);
+        if (sp1)
+          gram_split_slab(X, Y, l, h00, h10, h11);
+        else
+          gram_fp64_slab(X, Y, h00, h10, h11);
       }
     }
     if (w && w < nw) {
@@ -901,9 +909,9 @@ __device__ __forceinline__ void upgram_item(UpgLds &S, double *__restrict__ G, u
       h11 = h11 + S.red[1][2][v - 1u][l];
     }
     if (pn0 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, false, g00, g10, g11);
+      store_partial(part + ((size_t)Q.n0 * nchunk + c) * 768u, l, sp0, g00, g10, g11);
     if (pn1 != PLAN_SKIP)
-      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, false, h00, h10, h11);
+      store_partial(part + ((size_t)Q.n1 * nchunk + c) * 768u, l, sp1, h00, h10, h11);
   }
 }
 
@@ -911,12 +919,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPGRAM_WAVE
 void k_upgram(double *__restrict__ G, unsigned ldG, const uint16_t *__restrict__ pairs,
               const double *__restrict__ Wbuf, const unsigned *__restrict__ flags,
               const Quad *__restrict__ quads, unsigned nchunk, unsigned CH,
-              double *__restrict__ part, const uint8_t *__restrict__ plan_next)
+              double *__restrict__ part, const uint8_t *__restrict__ plan_next, int fsplit)
 {
   __shared__ UpgLds S;
   const unsigned qi = blockIdx.x / nchunk, c = blockIdx.x - qi * nchunk;
   const Quad Q = quads[qi];
-  upgram_item(S, G, ldG, pairs, Wbuf, flags, Q, c, nchunk, CH, part, plan_next, blockDim.x >> 6);
+  upgram_item(S, G, ldG, pairs, Wbuf, flags, Q, c, nchunk, CH, part, plan_next, blockDim.x >> 6,
+              fsplit != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -947,8 +956,10 @@ void k_stage(StageArgs A)
     const unsigned half = tid >> 7, ht = tid & 127u, k = 2u * b + half;
     const bool present = k < A.npl;
     const unsigned pi = present ? A.plist[k] : 0u;
-    // every Gram a staged launch's visits read comes from fused items (FP64)
-    const uint8_t plan = visit_plan((present && A.pplan) ? A.pplan[pi] : PLAN_FP64, false);
+    // every Gram a staged launch's visits read comes from fused items (split
+    // for split-planned pairs exactly when they form split Grams)
+    const uint8_t plan =
+        visit_plan((present && A.pplan) ? A.pplan[pi] : PLAN_FP64, A.fsplit != 0);
     pair_visit<SORT>(ht, present, pi, S.pair[half], &S.pair[half ^ 1u].go, A.ppairs, A.ppart,
                      A.nchunk, A.pW, A.pflags, A.ctl, A.prec, plan, S0, A.s0steps, A.nswp,
                      A.definite, A.nplus);
@@ -983,7 +994,7 @@ void k_stage(StageArgs A)
       // byte fields would be reloaded after every store of the item
       const Quad Q = A.quads[A.qord ? A.qord[qi] : qi];
       upgram_item(S.upg, A.G, A.ldG, A.upairs, A.uW, A.uflags, Q, c, A.nchunk, A.CH, A.upart,
-                  A.uplan, A.nw);
+                  A.uplan, A.nw, A.fsplit != 0);
       __syncthreads();
     }
   }
@@ -1208,11 +1219,11 @@ void launch_gram(hipStream_t s, const double *G, unsigned ldG, const uint16_t *p
 
 void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
                    const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
-                   unsigned CH, double *part, const uint8_t *plan_next)
+                   unsigned CH, double *part, const uint8_t *plan_next, bool fsplit)
 {
   const unsigned nw = gram_waves(CH);
   hipLaunchKernelGGL(k_upgram, dim3(nquads * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, W, flags,
-                     quads, nchunk, CH, part, plan_next);
+                     quads, nchunk, CH, part, plan_next, fsplit ? 1 : 0);
 }
 
 void launch_stage(hipStream_t s, bool sort, const StageArgs &a)

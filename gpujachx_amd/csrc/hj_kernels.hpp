@@ -86,7 +86,7 @@ bool closed_quads(const uint16_t *tab, unsigned steps, unsigned np, std::vector<
 
 void launch_upgram(hipStream_t s, double *G, unsigned ldG, const uint16_t *pairs, const double *W,
                    const unsigned *flags, const Quad *quads, unsigned nquads, unsigned nchunk,
-                   unsigned CH, double *part, const uint8_t *plan_next);
+                   unsigned CH, double *part, const uint8_t *plan_next, bool fsplit);
 
 // One launch of a staged sweep: the visits of a list of pairs of one step
 // (two per workgroup) beside the fused transition of a range of closed quads
@@ -119,6 +119,8 @@ struct StageArgs {
   unsigned niw;            // item workgroups (1..nquads*nchunk; 0 without items)
   double *upart;           // Gram partials of the next step
   const uint8_t *uplan;    // plans of the next step, or null
+  int fsplit;              // items form split Grams for split-planned pairs
+                           // (and the visits' Grams were formed that way)
   // update of V by the W of one step: (pair, row chunk) items, one per wave
   double *V;
   unsigned ldV, nrowV, chV, nchV;

@@ -182,6 +182,11 @@ void hj_set_visit_trace(void *host_buf, unsigned max_sweeps);
  * convergence test stay FP64, and a run ends only on a sweep that is all
  * FP64 (and unskipped) without a big rotation. */
 void hj_set_split_tau(float tau);
+/* 1: the fused block update + next-step Gram forms the split Gram of the
+ * pairs planned split, as the standalone Gram launch does; 0 (default): only
+ * the standalone Gram launches (step 0 of a sweep) do, the fused ones form
+ * FP64 Grams.  Returns the previous setting. */
+int hj_set_fused_split(int on);
 /* Split-Gram visits per sweep of the last call and their total. */
 void hj_last_splits(unsigned *split, unsigned long long *total);
 /* Self-test: the 32x32 Gram of columns 0..31 of a column-major nrow x 32

@@ -88,6 +88,30 @@ def test_split_form_matches_plain(alg, kind):
         assert np.abs(res).max() < 1e-12 * np.abs(G).max() * n
 
 
+def test_fused_split_grams():
+    # the fused block update + next-step Gram forms the split Gram of every
+    # split-planned pair: in the first sweep (every visit planned split) all
+    # visits then hold split Grams; with it off only the visits after a
+    # standalone Gram launch do
+    n = 512
+    G = svd_input(512, n, 10.0 ** (-10.0 * np.random.default_rng(4).uniform(size=n)), seed=44)
+    rp, _, _, _ = _run(g.FORM_PLAIN, 9, G, n)
+    rr, _, _, _ = _run(g.FORM_PLAIN, 9, G, n, snp1="rowcyc")
+    r0, sp0, tot0, rb0 = _run(g.FORM_SPLIT_GRAM, 9, G, n)
+    old = g.lib().hj_set_fused_split(1)
+    try:
+        r1, sp1, tot1, rb1 = _run(g.FORM_SPLIT_GRAM, 9, G, n)
+    finally:
+        g.lib().hj_set_fused_split(old)
+    assert old == 0
+    visits = (n // 32) * (n // 16 - 1)
+    assert sp1[0] == visits, (sp1[0], visits)
+    assert 0 < sp0[0] < visits and tot0 < tot1
+    for r, sp, rb in ((r0, sp0, rb0), (r1, sp1, rb1)):
+        assert r.ret == 0 and rb[-1] == 0 and sp[-1] == 0
+        assert_per_value(r.D, rp.D, rr.D, "split form, fused split Grams %s" % (r is r1))
+
+
 def test_split_form_nearly_dependent_columns():
     # two columns at an angle ~1e-6: their split Gram may lose positive
     # definiteness; such visits must fall back to FP64 and the result match

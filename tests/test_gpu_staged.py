@@ -103,7 +103,9 @@ def test_staged_not_used_where_it_does_not_apply():
     # the V update of the staged launches, a mixed J (NPLUS = N/2) and a
     # strategy whose cross transitions differ from cycwor's
     ("learned+split", 11, "svd", "cycwor"), ("learned+split", 9, "hyp", "cycwor"),
-    ("learned+split", 11, "hyp", "rowcyc"), ("learned+split", 9, "svd", "rowcyc")])
+    ("learned+split", 11, "hyp", "rowcyc"), ("learned+split", 9, "svd", "rowcyc"),
+    # split Grams formed by the fused items as well
+    ("split+fused", 9, "svd", "cycwor"), ("learned+split+fused", 11, "hyp", "cycwor")])
 def test_staged_forms_equal_step_by_step(form, alg, kind, snp1):
     # skipped and split visits inside staged launches (with the learned
     # skip: the sweep's visit lists and quads reordered by the plan): the
@@ -117,20 +119,24 @@ def test_staged_forms_equal_step_by_step(form, alg, kind, snp1):
     G = svd_input(n, n, s, seed=31)
     nplus = n if kind == "svd" else n // 2
     outs = []
-    for staged in (True, False):
-        if "learned" in form:
-            learned.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
-            g.lib().hj_set_skip_min_visits(0)
-            g.lib().hj_set_skip_min_propose(0.0)
-        else:
-            g.set_form(g.FORM_SPLIT_GRAM)
-        try:
-            r, used = run(alg, G, nplus, staged, snp1)
-            outs.append((r, used, g.last_skips()[1], g.last_splits()[1]))
-        finally:
-            g.set_form(g.FORM_PLAIN)
-            g.lib().hj_set_skip_min_visits(32000)
-            g.lib().hj_set_skip_min_propose(0.1)
+    old_fs = g.lib().hj_set_fused_split(1 if "fused" in form else 0)
+    try:
+        for staged in (True, False):
+            if "learned" in form:
+                learned.enable(form=g.FORM_LEARNED_SKIP | g.FORM_SPLIT_GRAM)
+                g.lib().hj_set_skip_min_visits(0)
+                g.lib().hj_set_skip_min_propose(0.0)
+            else:
+                g.set_form(g.FORM_SPLIT_GRAM)
+            try:
+                r, used = run(alg, G, nplus, staged, snp1)
+                outs.append((r, used, g.last_skips()[1], g.last_splits()[1]))
+            finally:
+                g.set_form(g.FORM_PLAIN)
+                g.lib().hj_set_skip_min_visits(32000)
+                g.lib().hj_set_skip_min_propose(0.1)
+    finally:
+        g.lib().hj_set_fused_split(old_fs)
     (a, ua, ska, spa), (b, ub, skb, spb) = outs
     assert ua and not ub
     assert (ska, spa) == (skb, spb)
