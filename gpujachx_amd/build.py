@@ -128,7 +128,7 @@ def build_oracle(ref=True):
     out = os.path.join(ORACLE, "libhjoracle.so")
     _run(["gcc", "-O2", "-mfma", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC",
           "-shared", "-Wall", "-o", out + ".tmp", os.path.join(ORACLE, "hjoracle.c"),
-          os.path.join(ORACLE, "hjoracle_qr.c"), "-lm"])
+          os.path.join(ORACLE, "hjoracle_qr.c"), os.path.join(ORACLE, "hjoracle_wide.c"), "-lm"])
     os.replace(out + ".tmp", out)
     if ref and os.path.isdir("/root/reference/strat"):
         _run(["bash", os.path.join(ORACLE, "build_ref.sh"), "/root/reference"])

@@ -50,6 +50,15 @@ def lib():
         L.hjo_hypJacL2.argtypes = [u, u, u, u, pd, u, pd, u, pd, pu, pull, pull,
                                    ctypes.c_void_p, u, ctypes.c_void_p, u, i, ctypes.c_long, pu, pu]
         L.hjo_hypJacL2.restype = i
+        L.hjo_hypJacL2_w.argtypes = [u, u, u, u, pd, u, pd, u, pd, pu, pull, pull, u,
+                                     ctypes.c_void_p, u, ctypes.c_void_p, u, u, i, pu]
+        L.hjo_hypJacL2_w.restype = i
+        L.hjo_gram_partial_w.argtypes = [pd, u, u, u, u, u, u, pd]
+        L.hjo_gram_partial_w.restype = None
+        L.hjo_pair_w.argtypes = [pd, u, u, i, i, i, u, ctypes.c_void_p, u, u, pd, pu, pi]
+        L.hjo_pair_w.restype = u
+        L.hjo_update_w.argtypes = [pd, u, u, u, u, u, pd]
+        L.hjo_update_w.restype = None
         L.hjo_set_max_sweeps.argtypes = [u]
         L.hjo_set_max_sweeps.restype = u
         L.hjo_set_single_wave_gram.argtypes = [i]
@@ -135,6 +144,81 @@ def hypJacL2(routine, G, nplus, strat0, strat1, ldhG=None, nthreads=0, max_steps
     return dict(ret=ret, G=hG[:, :nrow].T.copy(), V=None if hV is None else hV.T.copy(), D=hD,
                 sweeps=k, rot_s=gs.value, rot_b=gb.value,
                 rot_s_trace=list(ts[:k]), rot_b_trace=list(tb[:k]))
+
+
+def hypJacL2_w(routine, G, nplus, width, strat0, strat1, max_sweeps=30):
+    """The method with blocks of `width` (16 or 32) columns: strat0 uint8
+    [S0, width, 2] of order 2 width, strat1 uint16 [S1, ncol/(2 width), 2] of
+    order ncol / width.  Returns the dict of hypJacL2 (rot_b_trace too)."""
+    G = np.asarray(G, dtype=np.float64)
+    nrow, ncol = G.shape
+    hG = np.ascontiguousarray(G.T)
+    full = bool(routine & 2)
+    hV = np.zeros((ncol, ncol), dtype=np.float64) if full else None
+    hD = np.zeros(ncol, dtype=np.float64)
+    s0 = np.ascontiguousarray(strat0, dtype=np.uint8)
+    s1 = np.ascontiguousarray(strat1, dtype=np.uint16)
+    sw = ctypes.c_uint(0)
+    gs, gb = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    tb = (ctypes.c_uint * 256)()
+    ret = lib().hjo_hypJacL2_w(routine, nrow, ncol, nplus, _pd(hG), nrow, _pd(hV),
+                               ncol if full else 0, _pd(hD), ctypes.byref(sw), ctypes.byref(gs),
+                               ctypes.byref(gb), int(width), s0.ctypes.data, s0.shape[0],
+                               s1.ctypes.data, s1.shape[0], int(max_sweeps), 0, tb)
+    k = sw.value
+    return dict(ret=ret, G=hG.T.copy(), V=None if hV is None else hV.T.copy(), D=hD,
+                sweeps=k, rot_s=gs.value, rot_b=gb.value, rot_b_trace=list(tb[:k]))
+
+
+def gram_partial_w(G, width, a, b, r0, nr):
+    """Gram partial (tiles of the pair's lower triangle, MFMA layout) of
+    rows [r0, r0+nr) of the pair of width-column blocks a, b of G."""
+    hG = np.ascontiguousarray(np.asarray(G, dtype=np.float64).T)
+    nt = 2 * width // 16
+    out = np.zeros(nt * (nt + 1) // 2 * 256)
+    lib().hjo_gram_partial_w(_pd(hG), G.shape[0], int(width), int(a), int(b), int(r0), int(nr),
+                             _pd(out))
+    return out
+
+
+def tiles_to_lower(part, width):
+    """The pair's Gram (lower triangle, zeros above) from one partial."""
+    n2 = 2 * width
+    nt = n2 // 16
+    A = np.zeros((n2, n2))
+    tt = 0
+    for ti in range(nt):
+        for tj in range(ti + 1):
+            t = part[tt * 256:(tt + 1) * 256].reshape(64, 4)
+            for l in range(64):
+                for r in range(4):
+                    A[16 * ti + 4 * r + l // 16, 16 * tj + l % 16] = t[l, r]
+            tt += 1
+    return A
+
+
+def pair_w(partials, width, sort, hyp, definite, npos, strat0, nswp=1):
+    """One pair of width-column blocks from its Gram partials; (W, s, b, upd)."""
+    p = np.ascontiguousarray(partials, dtype=np.float64).reshape(-1)
+    n2 = 2 * width
+    nt = n2 // 16
+    nchunk = p.size // (nt * (nt + 1) // 2 * 256)
+    s0 = np.ascontiguousarray(strat0, dtype=np.uint8)
+    W = np.zeros(n2 * n2)
+    b = ctypes.c_uint(0)
+    upd = ctypes.c_int(0)
+    s = lib().hjo_pair_w(_pd(p), nchunk, int(width), int(sort), int(hyp), int(definite), int(npos),
+                         s0.ctypes.data, s0.shape[0], nswp, _pd(W), ctypes.byref(b),
+                         ctypes.byref(upd))
+    return W.reshape(n2, n2).T.copy(), s, b.value, upd.value
+
+
+def update_w(G, width, a, b, Wm):
+    """G with the pair (a, b) of width-column blocks multiplied by Wm."""
+    hG = np.ascontiguousarray(np.asarray(G, dtype=np.float64).T)
+    Wc = np.ascontiguousarray(np.asarray(Wm, dtype=np.float64).T)
+    lib().hjo_update_w(_pd(hG), G.shape[0], int(width), int(a), int(b), G.shape[0], _pd(Wc))
+    return hG.T.copy()
 
 
 def set_single_wave_gram(on):

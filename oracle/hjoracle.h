@@ -128,6 +128,21 @@ void hjo_update(double *G0, double *G1, unsigned ld, unsigned nr,
 void hjo_initD(double *G, unsigned ld, unsigned nrow, unsigned ncol,
                unsigned nplus, double *D);
 
+/* hjoracle_wide.c: the method with blocks of w = 16 or 32 columns (pairs of
+ * 2w columns); w = 16 equals hjo_hypJacL2 bit for bit */
+void hjo_gram_partial_w(const double *G, unsigned ld, unsigned w, unsigned a, unsigned b,
+                        unsigned r0, unsigned nr, double *out);
+unsigned hjo_pair_w(const double *partials, unsigned nchunk, unsigned w, int sort, int hyp,
+                    int definite, unsigned npos, const uint8_t *strat0, unsigned s0steps,
+                    unsigned nswp, double *W, unsigned *big, int *upd);
+void hjo_update_w(double *M, unsigned ld, unsigned w, unsigned a, unsigned b, unsigned nr,
+                  const double *W);
+int hjo_hypJacL2_w(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *G,
+                   unsigned ldG, double *V, unsigned ldV, double *D, unsigned *glbSwp,
+                   unsigned long long *glb_s, unsigned long long *glb_b, unsigned w,
+                   const uint8_t *strat0, unsigned s0steps, const uint16_t *strat1,
+                   unsigned s1steps, unsigned max_sweeps, int nthreads, unsigned *rot_b_trace);
+
 /* Whole level-2 driver, reference 1/HypJacL2.cu:13-202 (returns the same
  * argument codes).  strat0: [s0steps][16][2] uint8, strat1: [s1steps][n1/2][2]
  * uint16.  nthreads <= 0: all OpenMP threads.  max_steps > 0 stops after
