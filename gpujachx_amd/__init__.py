@@ -134,6 +134,8 @@ def lib():
     L.hj_set_split_tau.restype = None
     L.hj_set_fused_split.argtypes = [ctypes.c_int]
     L.hj_set_fused_split.restype = ctypes.c_int
+    L.hj_set_block_width.argtypes = [u]
+    L.hj_set_block_width.restype = u
     L.hj_last_splits.argtypes = [pu, pull]
     L.hj_last_splits.restype = None
     L.hj_dbg_gram.argtypes = [i, u, pd, pd]

@@ -49,7 +49,7 @@ def _hdf5_prefix():
 def build_library():
     os.makedirs(LIB, exist_ok=True)
     objs = []
-    for src in ["hj_kernels.hip", "hj_qr.hip", "hj_mlp.hip", "hj_batch.hip", "hj_host.cpp",
+    for src in ["hj_kernels.hip", "hj_qr.hip", "hj_mlp.hip", "hj_batch.hip", "hj_wide.hip", "hj_host.cpp",
                 "hj_strat.cpp"]:
         obj = os.path.join(LIB, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []

@@ -36,6 +36,8 @@ static Stats g_stats;
 static int g_profile = 0;
 static long g_step_limit = 0;
 static unsigned g_max_sweeps = HYPJAC_NSWEEP;  // level-2 sweep cap
+// block width: 16 (the reference's) or 32 (64-column pairs, plain FP64)
+static unsigned g_width = 16u;
 // forms and the pair-skip model
 static unsigned g_form = 0u;
 static bool g_model_ready = false;
@@ -780,15 +782,162 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   w.release();
 }
 
+// The level-2 run with blocks of 32 columns (hj_wide.hip): the strategy
+// families loaded by init_strats at the orders this width needs (64 for
+// level 0, ncol/32 for level 1, from the library's generators), then per
+// sweep every step's Gram partials, visits and block updates (G, and V with
+// ALG & 2), and the reference's stopping test (1/HypJacL2.cu:167).  Plain
+// FP64 only: the forms and the QR factorization act at width 16.
+static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
+                        unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
+                        unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t caller)
+{
+  hipStream_t s = nullptr;
+  HIP_CALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  {
+    hipEvent_t e;
+    HIP_CALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_CALL(hipEventRecord(e, caller));
+    HIP_CALL(hipStreamWaitEvent(s, e, 0));
+    HIP_CALL(hipEventDestroy(e));
+  }
+  const bool cdsort = (routine & HYPJAC_CDSORT_1) != 0u;
+  const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
+  const bool blk_ori = (routine & HYPJAC_BLK_ORI) != 0u;
+  const unsigned nswp = blk_ori ? 1u : HYPJAC_NSWEEP;
+  double *const V = full_svd ? dV : nullptr;
+  Strategy t0, t1;
+  const char *const f0 = (STRAT0 >= 1u && STRAT0 <= 6u) ? kFamilies[STRAT0 - 1u] : nullptr;
+  const char *const f1 = (STRAT1 >= 1u && STRAT1 <= 6u) ? kFamilies[STRAT1 - 1u] : nullptr;
+  if (!f0 || !generate(f0, 64u, t0) || t0.steps > 64u)
+    HJ_DIE("32-column blocks: no level-0 strategy of order 64 in the loaded family");
+  if (!f1 || !generate(f1, ncol / 32u, t1))
+    HJ_DIE("32-column blocks: no level-1 strategy of order ncol/32 in the loaded family");
+  const unsigned np = ncol / 64u, steps = t1.steps;
+  const unsigned nchunk = gram_nchunk(nrow, np), CH = nrow / nchunk;
+  std::vector<uint8_t> s0(t0.tab.begin(), t0.tab.end());
+  double *d_part = nullptr, *d_W = nullptr;
+  unsigned *d_flags = nullptr;
+  unsigned long long *d_cvg = nullptr, *h_cvg = nullptr;
+  uint8_t *d_s0 = nullptr;
+  uint16_t *d_s1 = nullptr;
+  HIP_CALL(hipMalloc(&d_part, sizeof(double) * (size_t)np * nchunk * wide_part_doubles()));
+  HIP_CALL(hipMalloc(&d_W, sizeof(double) * (size_t)np * 4096u));
+  HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np));
+  HIP_CALL(hipMalloc(&d_cvg, sizeof(unsigned long long)));
+  HIP_CALL(hipHostMalloc(&h_cvg, sizeof(unsigned long long)));
+  HIP_CALL(hipMalloc(&d_s0, s0.size()));
+  HIP_CALL(hipMalloc(&d_s1, sizeof(uint16_t) * t1.tab.size()));
+  HIP_CALL(hipMemcpyAsync(d_s0, s0.data(), s0.size(), hipMemcpyHostToDevice, s));
+  HIP_CALL(hipMemcpyAsync(d_s1, t1.tab.data(), sizeof(uint16_t) * t1.tab.size(),
+                          hipMemcpyHostToDevice, s));
+  const int definite = nplus ? ((nplus < ncol) ? 0 : +1) : -1;
+  if (V)
+    launch_initV(s, V, 0u, ncol, lddV);
+  std::vector<hipEvent_t> evs;
+  if (g_profile) {
+    evs.resize((size_t)4u * steps);
+    for (auto &e : evs) HIP_CALL(hipEventCreate(&e));
+  }
+  g_stats = Stats();
+  *glb_s = 0ull;
+  *glb_b = 0ull;
+  unsigned blk_swp = 0u;
+  long long swp_tim = 0;
+  stopwatch_reset(swp_tim);
+  while (blk_swp < g_max_sweeps) {
+    HIP_CALL(hipMemsetAsync(d_cvg, 0, sizeof(unsigned long long), s));
+    for (unsigned st = 0u; st < steps; ++st) {
+      const uint16_t *const pairs = d_s1 + (size_t)st * np * 2u;
+      hipEvent_t *const ev = g_profile ? evs.data() + (size_t)4u * st : nullptr;
+      if (ev) HIP_CALL(hipEventRecord(ev[0], s));
+      launch_gram_w(s, dG, lddG, pairs, np, nchunk, CH, d_part);
+      if (ev) HIP_CALL(hipEventRecord(ev[1], s));
+      launch_pair_w(s, cdsort, pairs, np, d_part, nchunk, d_W, d_flags, d_cvg, d_s0, t0.steps,
+                    nswp, definite, nplus);
+      if (ev) HIP_CALL(hipEventRecord(ev[2], s));
+      launch_update_w(s, dG, lddG, nrow, pairs, np, d_W, d_flags);
+      if (V)
+        launch_update_w(s, V, lddV, ncol, pairs, np, d_W, d_flags);
+      if (ev) HIP_CALL(hipEventRecord(ev[3], s));
+    }
+    HIP_CALL(hipMemcpyAsync(h_cvg, d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    if (g_profile)
+      for (unsigned st = 0u; st < steps; ++st)
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0.0f;
+          HIP_CALL(hipEventElapsedTime(&ms, evs[(size_t)4u * st + k], evs[(size_t)4u * st + k + 1]));
+          g_stats.kms[k] += ms;
+          g_stats.kcnt[k] += 1ull;
+        }
+    ++blk_swp;
+    const unsigned cvg_s = static_cast<unsigned>(*h_cvg);
+    const unsigned cvg_b = static_cast<unsigned>(*h_cvg >> 32u);
+    *glb_s += cvg_s;
+    *glb_b += cvg_b;
+    const double tim_s = stopwatch_lap(swp_tim) * TS2S;
+    if (blk_swp <= HJ_MAX_SWEEP_CAP) {
+      g_stats.rot_s[blk_swp - 1u] = cvg_s;
+      g_stats.rot_b[blk_swp - 1u] = cvg_b;
+      g_stats.sec[blk_swp - 1u] = tim_s;
+    }
+    g_stats.nswp = blk_swp;
+    g_stats.visits += (unsigned long long)np * steps;
+    if (!quiet()) {
+      (void)fprintf(stdout, "BLK_SWP(%2u), ROT_S(%10u), ROT_B(%10u), TIME(%#12.6f s)\n", blk_swp,
+                    cvg_s, cvg_b, tim_s);
+      (void)fflush(stdout);
+    }
+    if (!cvg_b)
+      break;
+  }
+  for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
+  *glbSwp = blk_swp;
+  launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
+  HIP_CALL(hipStreamSynchronize(s));
+  HIP_CALL(hipStreamDestroy(s));
+  HIP_CALL(hipFree(d_part));
+  HIP_CALL(hipFree(d_W));
+  HIP_CALL(hipFree(d_flags));
+  HIP_CALL(hipFree(d_cvg));
+  HIP_CALL(hipHostFree(h_cvg));
+  HIP_CALL(hipFree(d_s0));
+  HIP_CALL(hipFree(d_s1));
+}
+
+// whether the families loaded by init_strats have the tables the 32-column
+// blocks need (orders 64 and ncol/32)
+static bool wide_tables_ok(unsigned ncol)
+{
+  if (STRAT0 < 1u || STRAT0 > 6u || STRAT1 < 1u || STRAT1 > 6u)
+    return false;
+  return can_generate(kFamilies[STRAT0 - 1u], 64u) && family_steps(kFamilies[STRAT0 - 1u], 64u) <= 64u &&
+         can_generate(kFamilies[STRAT1 - 1u], ncol / 32u);
+}
+
+// the level-2 run at the block width in force
+static void level2_run(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
+                       unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
+                       unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t caller)
+{
+  if (g_width == 32u)
+    level2_wide(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b, caller);
+  else
+    level2_core(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b, caller);
+}
+
 // argument checks of 1/HypJacL2.cu:39-77
+// (with blocks of `width` columns ncol must be a multiple of 2 width)
 static int check_args(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus,
                       const void *G, unsigned ldG, const void *V, unsigned ldV, const void *D,
-                      const void *glbSwp, const void *glb_s, const void *glb_b)
+                      const void *glbSwp, const void *glb_s, const void *glb_b,
+                      unsigned width = 16u)
 {
   if (routine >= 16u) return -1;
   const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
   if (!nrow || (nrow % 64u)) return -2;
-  if (!ncol || (ncol > nrow) || (ncol % 32u)) return -3;
+  if (!ncol || (ncol > nrow) || (ncol % (2u * width))) return -3;
   if (nplus > ncol) return -4;
   if (!G) return -5;
   if (ldG < nrow) return -6;
@@ -846,9 +995,12 @@ extern "C" int hypJacL2(const unsigned routine, const unsigned nrow, const unsig
 {
   long long timers[4] = {0ll, 0ll, 0ll, 0ll};
   stopwatch_reset(timers[0]);
-  const int rc = check_args(routine, nrow, ncol, nplus, hG, ldhG, hV, ldhV, hD, glbSwp, glb_s, glb_b);
+  const int rc = check_args(routine, nrow, ncol, nplus, hG, ldhG, hV, ldhV, hD, glbSwp, glb_s, glb_b,
+                            g_width);
   if (rc)
     return rc;
+  if (g_width == 32u && !wide_tables_ok(ncol))
+    return -3;
   stopwatch_reset(timers[3]);
   const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
   const unsigned lddG = nrow, lddV = full_svd ? ncol : 0u;
@@ -864,7 +1016,7 @@ extern "C" int hypJacL2(const unsigned routine, const unsigned nrow, const unsig
   HIP_CALL(hipStreamSynchronize(s));
   timers[1] = stopwatch_lap(timers[3]);
 
-  level2_core(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b, s);
+  level2_run(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b, s);
   timers[2] = stopwatch_lap(timers[3]);
 
   HIP_CALL(hipMemcpy2DAsync(hG, ldhG * sizeof(double), dG, lddG * sizeof(double),
@@ -893,15 +1045,18 @@ extern "C" int hypJacL2_dev(unsigned routine, unsigned nrow, unsigned ncol, unsi
 {
   long long t0 = 0, t1 = 0;
   stopwatch_reset(t0);
-  const int rc = check_args(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b);
+  const int rc = check_args(routine, nrow, ncol, nplus, dG, lddG, dV, lddV, dD, glbSwp, glb_s, glb_b,
+                            g_width);
   if (rc)
     return rc;
+  if (g_width == 32u && !wide_tables_ok(ncol))
+    return -3;
   const bool full_svd = (routine & HYPJAC_FULL_SVD) != 0u;
   if ((lddG & 1u) || (reinterpret_cast<uintptr_t>(dG) & 15u)) return -13;
   if (full_svd && ((lddV & 1u) || (reinterpret_cast<uintptr_t>(dV) & 15u))) return -13;
   hipStream_t s = static_cast<hipStream_t>(stream);
   stopwatch_reset(t1);
-  level2_core(routine, nrow, ncol, nplus, dG, lddG, full_svd ? dV : nullptr, lddV, dD, glbSwp,
+  level2_run(routine, nrow, ncol, nplus, dG, lddG, full_svd ? dV : nullptr, lddV, dD, glbSwp,
               glb_s, glb_b, s);
   const long long tc = stopwatch_lap(t1);
   const long long tw = stopwatch_lap(t0);
@@ -1236,6 +1391,14 @@ extern "C" int hj_set_factorization(int f)
   if (f == HJ_FACT_CHOLESKY || f == HJ_FACT_QR || f == HJ_FACT_QR_DRMAC)
     g_fact = f;
   return prev;
+}
+
+extern "C" unsigned hj_set_block_width(unsigned width)
+{
+  const unsigned old = g_width;
+  if (width == 16u || width == 32u)
+    g_width = width;
+  return old;
 }
 
 extern "C" unsigned hj_set_max_sweeps(unsigned n)

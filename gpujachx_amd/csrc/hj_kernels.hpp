@@ -165,6 +165,21 @@ void launch_update(hipStream_t s, double *G, unsigned ldG, unsigned nrowG, unsig
 void launch_initD(hipStream_t s, double *G, double *D, unsigned ifc, unsigned nRow, unsigned nRank,
                   unsigned nPlus, unsigned ldG);
 void launch_initV(hipStream_t s, double *V, unsigned ifc, unsigned nRank, unsigned ldV);
+
+// The wide-block form (hj_wide.hip): blocks of 32 columns, 64-column pairs;
+// Gram partials of wide_part_doubles() doubles per (pair, row chunk), W of
+// 64x64 per pair, the update in wide_update_chunks(nrow) workgroups a pair.
+unsigned wide_part_doubles();
+unsigned wide_update_chunks(unsigned nrow);
+void launch_gram_w(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
+                   unsigned np, unsigned nchunk, unsigned CH, double *part);
+void launch_pair_w(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np,
+                   const double *part, unsigned nchunk, double *W, unsigned *flags,
+                   unsigned long long *cvg, const uint8_t *strat0, unsigned s0steps,
+                   unsigned nswp, int definite, unsigned nplus);
+void launch_update_w(hipStream_t s, double *M, unsigned ldM, unsigned nrow,
+                     const uint16_t *pairs, unsigned np, const double *W,
+                     const unsigned *flags);
 void launch_sweep_begin(hipStream_t s, RunCtl *ctl);
 void launch_dbg_scalar(hipStream_t s, int kind, int n, const double *a, const double *b,
                        const double *c, double *oc, double *ot, int *ob);
