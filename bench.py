@@ -14,7 +14,9 @@ oracle, is timed beside it on the same matrix (`plain`).  Accuracy is
 reported per singular value against the plain FP64 run, next to the spread
 between two FP64 runs that differ only in their strategy (cycwor against
 rowcyc), and the same for the hyperbolic SVD of the same matrix with
-NPLUS = N/2 (`hsvd`).  The CPU baseline is LAPACK dgesvj (J = I) on all
+NPLUS = N/2 (`hsvd`).  `wide32`: the plain FP64 method on 32-column blocks
+(64-column block pairs, step-by-step launches), SVD and HSVD, against the
+plain form at the reference's 16.  The CPU baseline is LAPACK dgesvj (J = I) on all
 host cores, measured at two smaller orders of the same construction and
 extrapolated to N with the measured growth.
 
@@ -163,6 +165,8 @@ def main():
     ap.add_argument("--no-spread", action="store_true",
                     help="skip the FP64-vs-FP64 spread run (plain rowcyc)")
     ap.add_argument("--no-hsvd", action="store_true", help="skip the NPLUS=N/2 sub-record")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="skip the 32-column-block (plain FP64) sub-record")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the profiled run (HIP events around every launch) that gives the "
                          "per-kernel breakdown")
@@ -221,11 +225,13 @@ def main():
         if rc:
             raise RuntimeError("hypJacL2_dev returned %d" % rc)
 
-    def timed(form, steps, warmup, profile=False, nplus=None):
+    def timed(form, steps, warmup, profile=False, nplus=None, width=16):
         """Wall time per call of `form` over `steps` runs after `warmup`; with
-        profile, per-kernel HIP-event durations over the same runs."""
+        profile, per-kernel HIP-event durations over the same runs; width:
+        the block width (16, or 32 for 64-column block pairs)."""
         nplus = n if nplus is None else nplus
         set_form(form)
+        L.hj_set_block_width(width)
         for _ in range(warmup):
             one(nplus)
         g.set_profile(profile)
@@ -244,12 +250,14 @@ def main():
         elapsed = time.perf_counter() - t0
         g.set_profile(False)
         g.set_form(g.FORM_PLAIN)
+        L.hj_set_block_width(16)
         rot_s, rot_b, _ = g.last_sweeps()
         skips = g.last_skips()
         splits = g.last_splits()
         Dh = D.cpu().numpy().copy()
-        print("bench: %s nplus=%d %d run(s) %.3f s each, %d sweeps" % (form, nplus, steps,
-              elapsed / steps, sw.value), file=sys.stderr, flush=True)
+        print("bench: %s%s nplus=%d %d run(s) %.3f s each, %d sweeps"
+              % (form, " width 32" if width == 32 else "", nplus, steps, elapsed / steps,
+                 sw.value), file=sys.stderr, flush=True)
         return {"per": elapsed / steps, "kms": kms, "kcnt": kcnt, "visits": visits,
                 "staged": g.last_staged(),
                 "sweeps": sw.value, "rot_b": rot_b, "skipped": skips[1], "split": splits[1],
@@ -289,6 +297,22 @@ def main():
         g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
         hsvd["fp64_spread_cycwor_vs_rowcyc"] = per_sigma(hs["D"], hp["D"])
         hsvd["plain_rowcyc"] = {"s": hs["per"], "sweeps": hs["sweeps"]}
+
+    wide = None
+    if not args.no_wide and cmp_run is not None and n % 64 == 0:
+        # the plain FP64 method on 32-column blocks (64-column pairs), step by
+        # step, against the plain form at the reference's 16 (cmp_run)
+        wp = timed("plain", 1, 0, width=32)
+        wide = {"block_width": 32, "form": "plain",
+                "s": wp["per"], "sweeps": wp["sweeps"],
+                "plain_width16": {"s": cmp_run["per"], "sweeps": cmp_run["sweeps"]},
+                "speedup_vs_plain_width16": cmp_run["per"] / wp["per"],
+                "per_sigma_vs_plain_width16": per_sigma(wp["D"], cmp_run["D"])}
+        if hsvd is not None:
+            wh = timed("plain", 1, 0, nplus=n // 2, width=32)
+            wide["hsvd"] = {"NPLUS": n // 2, "s": wh["per"], "sweeps": wh["sweeps"],
+                            "speedup_vs_plain_width16": hsvd["plain"]["s"] / wh["per"],
+                            "per_sigma_vs_plain_width16": per_sigma(wh["D"], hp["D"])}
 
     per = main_run["per"]
     pk = prof_run if prof_run is not None else main_run
@@ -450,6 +474,8 @@ def main():
                                 "reaches on this matrix (kappa ~ 1e12)")
     if hsvd is not None:
         out["hsvd"] = hsvd
+    if wide is not None:
+        out["wide32"] = wide
     if not args.no_cpu:
         cb = cpu_baseline(n, args.seed)
         if cb is not None:

@@ -11,7 +11,9 @@ than the headline one, for the plain FP64 form and the learned+split form.
           [1e-10, 1], column grading u_j ~ U(-4, 4) as in c2; ALG 9
   c5hsvd  the same matrix with NPLUS = N/2; ALG 9
 
-    python -m gpujachx_amd.config_bench c1 c2 [--forms plain learned+split]
+    python -m gpujachx_amd.config_bench c1 c2 [--forms plain learned+split plain32]
+
+(plain32: the plain FP64 method on 32-column blocks, 64-column pairs.)
 
 One JSON line per (configuration, form): s per call (device-resident
 input, after one warm-up call for the small configurations), sweeps,
@@ -137,7 +139,14 @@ def main():
         g.init_strats_builtin("cycwor", 32, "cycwor", cfg["n"] // 16)
         base = None
         for form in args.forms:
-            t, W, V, D, sweeps, rb, sk = run(g, torch, cfg, Gt, form, args.model)
+            # plain32: the plain FP64 method on 32-column blocks
+            width = 32 if form == "plain32" else 16
+            g.lib().hj_set_block_width(width)
+            try:
+                t, W, V, D, sweeps, rb, sk = run(g, torch, cfg, Gt,
+                                                 "plain" if width == 32 else form, args.model)
+            finally:
+                g.lib().hj_set_block_width(16)
             sig = torch.sqrt(D.abs())
             out = {"config": name, "form": form, "M": cfg["m"], "N": cfg["n"], "NPLUS": cfg["nplus"],
                    "alg": cfg["alg"], "s_per_call": t, "sweeps": sweeps,
