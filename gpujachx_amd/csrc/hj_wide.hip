@@ -41,44 +41,64 @@ __device__ __forceinline__ unsigned tile_i(unsigned tt)
 // Gram partials of the pairs of one step: workgroup (pair, row chunk), its
 // waves taking equal row sub-chunks, wave 0 adding the others' tiles in
 // wave order; partial of (pair, chunk) at part[(pi * nchunk + c) * WPART],
-// tile tt at + 256 tt, element (4r + l/16, l%16) at + 4l + r.
+// tile tt at + 256 tt, element (4r + l/16, l%16) at + 4l + r.  A slab (16
+// rows x 64 columns) is read 128 bytes of a column per lane group (lane l:
+// row l%16 of column 4u + l/16) with the next slab's loads in flight, and
+// turned into the MFMA operand layout (lane l, register r of group t: row
+// 4r + l/16 of column 16t + l%16) through the wave's LDS buffer.
+constexpr unsigned LDTW = 17u;  // padded column of a wave's slab buffer
+
+struct GramLdsW {
+  union {
+    d4 red[GRAM_WAVES - 1][WTILES][64];
+    double T[GRAM_WAVES][64u * LDTW];
+  };
+};
+
 __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, unsigned ldG,
                                                 const uint16_t *__restrict__ pairs,
                                                 unsigned nchunk, unsigned CH,
                                                 double *__restrict__ part)
 {
-  __shared__ d4 red[GRAM_WAVES - 1][WTILES][64];
+  __shared__ GramLdsW S;
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const unsigned pi = blockIdx.x / nchunk, c = blockIdx.x - pi * nchunk;
   const unsigned a = pairs[2u * pi], b = pairs[2u * pi + 1u];
   const unsigned SUB = CH / nw;
-  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + (l >> 4);
-  const double *g[4];
-#pragma unroll
-  for (unsigned t = 0u; t < 4u; ++t)
-    g[t] = G + (size_t)(32u * ((t < 2u) ? a : b) + 16u * (t & 1u) + (l & 15u)) * ldG + r0;
+  const size_t r0 = (size_t)c * CH + (size_t)w * SUB + (l & 15u);
+  // column 4u + l/16 of the pair: block a for u < 8, block b after
+  const double *const ga = G + (size_t)(32u * a + (l >> 4)) * ldG + r0;
+  const double *const gb = G + (size_t)(32u * b + (l >> 4)) * ldG + r0;
+  lds_f64 *const T = (lds_f64 *)S.T[w];
   d4 acc[WTILES];
 #pragma unroll
   for (unsigned tt = 0u; tt < WTILES; ++tt)
     acc[tt] = d4{0.0, 0.0, 0.0, 0.0};
-  d4 nx[4];
+  double xn[16];
 #pragma unroll
-  for (unsigned t = 0u; t < 4u; ++t)
-#pragma unroll
-    for (unsigned r = 0u; r < 4u; ++r)
-      nx[t][r] = g[t][4u * r];
+  for (unsigned u = 0u; u < 8u; ++u) {
+    xn[u] = ga[(size_t)(4u * u) * ldG];
+    xn[8u + u] = gb[(size_t)(4u * u) * ldG];
+  }
   for (unsigned s = 0u; s < SUB; s += 16u) {
+#pragma unroll
+    for (unsigned u = 0u; u < 16u; ++u)
+      T[(4u * u + (l >> 4)) * LDTW + (l & 15u)] = xn[u];
+    if (s + 16u < SUB) {
+#pragma unroll
+      for (unsigned u = 0u; u < 8u; ++u) {
+        xn[u] = ga[(size_t)(4u * u) * ldG + s + 16u];
+        xn[8u + u] = gb[(size_t)(4u * u) * ldG + s + 16u];
+      }
+    }
+    wave_lds_sync();
     d4 X[4];
 #pragma unroll
     for (unsigned t = 0u; t < 4u; ++t)
-      X[t] = nx[t];
-    if (s + 16u < SUB) {
 #pragma unroll
-      for (unsigned t = 0u; t < 4u; ++t)
-#pragma unroll
-        for (unsigned r = 0u; r < 4u; ++r)
-          nx[t][r] = g[t][s + 16u + 4u * r];
-    }
+      for (unsigned r = 0u; r < 4u; ++r)
+        X[t][r] = T[(16u * t + (l & 15u)) * LDTW + 4u * r + (l >> 4)];
+    wave_lds_sync();
 #pragma unroll
     for (unsigned r = 0u; r < 4u; ++r) {
       unsigned tt = 0u;
@@ -89,10 +109,11 @@ __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, un
           acc[tt] = mfma(X[ti][r], X[tj][r], acc[tt]);
     }
   }
+  __syncthreads();  // the slab buffers become the reduction buffer
   if (w) {
 #pragma unroll
     for (unsigned tt = 0u; tt < WTILES; ++tt)
-      red[w - 1u][tt][l] = acc[tt];
+      S.red[w - 1u][tt][l] = acc[tt];
   }
   __syncthreads();
   if (w == 0u) {
@@ -101,7 +122,7 @@ __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, un
     for (unsigned tt = 0u; tt < WTILES; ++tt) {
       d4 v = acc[tt];
       for (unsigned u = 1u; u < nw; ++u)
-        v = v + red[u - 1u][tt][l];
+        v = v + S.red[u - 1u][tt][l];
       *reinterpret_cast<d4 *>(o + tt * 256u + 4u * l) = v;
     }
   }
