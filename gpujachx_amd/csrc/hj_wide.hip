@@ -10,10 +10,11 @@
 //     16-row slabs, the waves added in order, the chunks in order by the
 //     visit;
 //   - Cholesky (share/device_code_common_Cholesky.hpp:58-158, USE_RSQRT) at
-//     64 pivots by one wave, one row per lane;
-//   - level-0: 32 pivot pairs a step, 8 lanes a pair, lane r holding rows
-//     r + 8i (i < 8) of both columns of R and W; dot products by the xor
-//     butterfly 32,16,8 in the lane, 4,2,1 across lanes; the convergence
+//     64 pivots, one row per lane, the pivots on one wave and the column
+//     updates on the others;
+//   - level-0: 32 pivot pairs a step, 16 lanes a pair, lane r holding rows
+//     r + 16i (i < 4) of both columns of R and W; dot products by the xor
+//     butterfly 32,16 in the lane, 8,4,2,1 across lanes; the convergence
 //     test at tolerance sqrt(64) 2^-53;
 //   - update: out(r, j) = MFMA chain over k = 0..63.
 #include "../../include/hypjac.h"
@@ -128,24 +129,29 @@ __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, un
   }
 }
 
-// Sum of the 64 rows of one pivot pair held by 8 lanes, lane r holding rows
-// r + 8i as v[i]: the xor butterfly 32,16,8 inside the lane, 4,2,1 across.
-__device__ __forceinline__ double sum8x8(const double *v)
+// Sum of the 64 rows of one pivot pair held by 16 lanes, lane r holding
+// rows r + 16i as v[i]: the xor butterfly 32,16 inside the lane, 8 (a
+// rotation by 8 inside the 16-lane DPP row), 4,2,1 across lanes.
+__device__ __forceinline__ double lane_xor8(double v) { return dpp_mov<0x128>(v); }
+
+__device__ __forceinline__ double sum16x4(const double *v)
 {
-  const double a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
-  double s = (a0 + a2) + (a1 + a3);
+  double s = (v[0] + v[2]) + (v[1] + v[3]);
+  s = s + lane_xor8(s);
   s = s + lane_xor4(s);
   s = s + lane_xor2(s);
   s = s + lane_xor1(s);
   return s;
 }
 
-// In-place 64x64 Cholesky by the workgroup's four waves, in LDS (lane x
+constexpr unsigned VT = 512u;  // threads of a visit: 32 pivot pairs x 16 lanes
+
+// In-place 64x64 Cholesky by the workgroup's eight waves, in LDS (lane x
 // of every wave standing for row x of the lower triangle):
 // share/device_code_common_Cholesky.hpp:58-158 (USE_RSQRT).  Per pivot k,
 // wave 0 scales column k (rsqrt(A(k,k)) below the diagonal, sqrt(A(k,k)) on
 // it), copies it to col and forms the next pivot A(k+1,k+1) - L(k+1,k)^2;
-// after a barrier it computes that pivot's rsqrt and sqrt while waves 1-3
+// after a barrier it computes that pivot's rsqrt and sqrt while waves 1-7
 // apply A(x,j) = fma(-A(x,k), A(j,k), A(x,j)) to the columns j > k in
 // batches of 8 (a batch's loads, then its fmas, then its stores; rows above
 // j produce values never read).  Then R = L^T, zeros below the diagonal.
@@ -181,7 +187,7 @@ __device__ __forceinline__ void chol64_block(double *A_, double *col_, unsigned 
       }
       else {
         const double v = col[x];
-        for (unsigned jb = (k1 & ~7u) + 8u * (w - 1u); jb < 64u; jb += 24u) {
+        for (unsigned jb = (k1 & ~7u) + 8u * (w - 1u); jb < 64u; jb += 8u * (VT / 64u - 1u)) {
           double av[8], cv[8];
 #pragma unroll
           for (unsigned i = 0u; i < 8u; ++i) {
@@ -200,16 +206,16 @@ __device__ __forceinline__ void chol64_block(double *A_, double *col_, unsigned 
     __syncthreads();
   }
   // R(r, c) = L(c, r) for r <= c: all reads, a barrier, all writes
-  double t[16];
+  double t[4096u / VT];
 #pragma unroll
-  for (unsigned i = 0u; i < 16u; ++i) {
-    const unsigned e = tid + 256u * i, r = e & 63u, c = e >> 6;
+  for (unsigned i = 0u; i < 4096u / VT; ++i) {
+    const unsigned e = tid + VT * i, r = e & 63u, c = e >> 6;
     t[i] = (r <= c) ? A[r * LDRW + c] : 0.0;
   }
   __syncthreads();
 #pragma unroll
-  for (unsigned i = 0u; i < 16u; ++i) {
-    const unsigned e = tid + 256u * i, r = e & 63u, c = e >> 6;
+  for (unsigned i = 0u; i < 4096u / VT; ++i) {
+    const unsigned e = tid + VT * i, r = e & 63u, c = e >> 6;
     A[c * LDRW + r] = t[i];
   }
 }
@@ -225,11 +231,11 @@ struct PairLdsW {
   uint8_t S0[S0W_BYTES];
 };
 
-// One 64-column block pair per workgroup of 256 threads: chunk partials ->
+// One 64-column block pair per workgroup of 512 threads: chunk partials ->
 // R (lower), Cholesky, level-0 sweeps on R accumulating W, counters, W and
 // the update flag out.  The arithmetic of pair_visit at 64 columns.
 template <bool SORT>
-__global__ __launch_bounds__(256) void k_pair_w(const uint16_t *__restrict__ pairs,
+__global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pairs,
                                                 const double *__restrict__ part, unsigned nchunk,
                                                 double *__restrict__ Wout,
                                                 unsigned *__restrict__ flags,
@@ -258,23 +264,23 @@ __global__ __launch_bounds__(256) void k_pair_w(const uint16_t *__restrict__ pai
   }
   if (tid == 0u)
     L.swp = 0;
-  for (unsigned e = tid; e < 2u * 32u * s0steps; e += 256u)
+  for (unsigned e = tid; e < 2u * 32u * s0steps; e += VT)
     L.S0[e] = strat0[e];
   {
     const double *const pp = part + (size_t)pi * nchunk * WPART;
     // two entries a thread at a time, all their chunk loads in flight
     // together; the chunks summed in chunk order
 #pragma unroll 1
-    for (unsigned q = 0u; q < WTILES; q += 2u) {
-      double v[2][MAX_GRAM_CHUNKS];
+    for (unsigned q = 0u; q < WPART / VT; ++q) {
+      double v[1][MAX_GRAM_CHUNKS];
 #pragma unroll
-      for (unsigned h = 0u; h < 2u; ++h)
+      for (unsigned h = 0u; h < 1u; ++h)
 #pragma unroll
         for (unsigned c = 0u; c < MAX_GRAM_CHUNKS; ++c)
-          if (c < nchunk) v[h][c] = pp[(size_t)c * WPART + tid + 256u * (q + h)];
+          if (c < nchunk) v[h][c] = pp[(size_t)c * WPART + tid + VT * (q + h)];
 #pragma unroll
-      for (unsigned h = 0u; h < 2u; ++h) {
-        const unsigned e = tid + 256u * (q + h);
+      for (unsigned h = 0u; h < 1u; ++h) {
+        const unsigned e = tid + VT * (q + h);
         double s = v[h][0];
 #pragma unroll
         for (unsigned c = 1u; c < MAX_GRAM_CHUNKS; ++c)
@@ -285,35 +291,35 @@ __global__ __launch_bounds__(256) void k_pair_w(const uint16_t *__restrict__ pai
       }
     }
   }
-  for (unsigned e = tid; e < 4096u; e += 256u)
+  for (unsigned e = tid; e < 4096u; e += VT)
     RW(W, e & 63u, e >> 6) = ((e & 63u) == (e >> 6)) ? 1.0 : 0.0;
   __syncthreads();
   chol64_block(R, L.col, tid);
   __syncthreads();
 
-  const unsigned y = tid >> 3, r = tid & 7u;
+  const unsigned y = tid >> 4, r = tid & 15u;
   unsigned blk_s = 0u, blk_b = 0u;
   bool swapped = false;
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned my_s = 0u, my_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
       const unsigned p = L.S0[(step * 32u + y) * 2u], q = L.S0[(step * 32u + y) * 2u + 1u];
-      double Gp[8], Gq[8], Vp[8], Vq[8], t0[8], t1[8], t2[8];
+      double Gp[4], Gq[4], Vp[4], Vq[4], t0[4], t1[4], t2[4];
 #pragma unroll
-      for (unsigned i = 0u; i < 8u; ++i) {
-        Gp[i] = RW(R, r + 8u * i, p);
-        Gq[i] = RW(R, r + 8u * i, q);
-        Vp[i] = RW(W, r + 8u * i, p);
-        Vq[i] = RW(W, r + 8u * i, q);
+      for (unsigned i = 0u; i < 4u; ++i) {
+        Gp[i] = RW(R, r + 16u * i, p);
+        Gq[i] = RW(R, r + 16u * i, q);
+        Vp[i] = RW(W, r + 16u * i, p);
+        Vq[i] = RW(W, r + 16u * i, q);
       }
 #pragma unroll
-      for (unsigned i = 0u; i < 8u; ++i) {
+      for (unsigned i = 0u; i < 4u; ++i) {
         t0[i] = __fma_rn(Gp[i], Gp[i], 0.0);
         t1[i] = __fma_rn(Gq[i], Gq[i], 0.0);
         t2[i] = __fma_rn(Gp[i], Gq[i], 0.0);
       }
-      double Dp = sum8x8(t0), Dq = sum8x8(t1);
-      const double Apq = sum8x8(t2);
+      double Dp = sum16x4(t0), Dq = sum16x4(t1);
+      const double Apq = sum16x4(t2);
       const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
       const int transf_s = !(Apq_ < (Dpq_ * WTOL));
       int pdef = def;
@@ -340,22 +346,22 @@ __global__ __launch_bounds__(256) void k_pair_w(const uint16_t *__restrict__ pai
           keep = (!pdef) || ((pdef > 0) && (Dp >= Dq)) || ((pdef < 0) && (Dp <= Dq));
         const unsigned P = keep ? p : q, Q = keep ? q : p;
 #pragma unroll
-        for (unsigned i = 0u; i < 8u; ++i) {
+        for (unsigned i = 0u; i < 4u; ++i) {
           const double rp = __fma_rn(t_, Gq[i], Gp[i]), rq = __fma_rn(t, Gp[i], Gq[i]);
           const double wp = __fma_rn(t_, Vq[i], Vp[i]), wq = __fma_rn(t, Vp[i], Vq[i]);
-          RW(R, r + 8u * i, P) = c * rp;
-          RW(R, r + 8u * i, Q) = c * rq;
-          RW(W, r + 8u * i, P) = c * wp;
-          RW(W, r + 8u * i, Q) = c * wq;
+          RW(R, r + 16u * i, P) = c * rp;
+          RW(R, r + 16u * i, Q) = c * rq;
+          RW(W, r + 16u * i, P) = c * wp;
+          RW(W, r + 16u * i, Q) = c * wq;
         }
       }
       else if (SORT && (((pdef > 0) && (Dp < Dq)) || ((pdef < 0) && (Dq < Dp)))) {
 #pragma unroll
-        for (unsigned i = 0u; i < 8u; ++i) {
-          RW(R, r + 8u * i, p) = Gq[i];
-          RW(R, r + 8u * i, q) = Gp[i];
-          RW(W, r + 8u * i, p) = Vq[i];
-          RW(W, r + 8u * i, q) = Vp[i];
+        for (unsigned i = 0u; i < 4u; ++i) {
+          RW(R, r + 16u * i, p) = Gq[i];
+          RW(R, r + 16u * i, q) = Gp[i];
+          RW(W, r + 16u * i, p) = Vq[i];
+          RW(W, r + 16u * i, q) = Vp[i];
         }
         swapped = true;
       }
@@ -393,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pair_w(const uint16_t *__restrict__ pai
     flags[pi] = (blk_s || L.swp) ? 1u : 0u;
   }
   double *const wo = Wout + (size_t)pi * 4096u;
-  for (unsigned e = tid; e < 4096u; e += 256u)
+  for (unsigned e = tid; e < 4096u; e += VT)
     wo[e] = RW(W, e & 63u, e >> 6);
 }
 
@@ -486,10 +492,10 @@ void launch_pair_w(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np,
                    unsigned nswp, int definite, unsigned nplus)
 {
   if (sort)
-    hipLaunchKernelGGL(k_pair_w<true>, dim3(np), dim3(256), 0, s, pairs, part, nchunk, W, flags,
+    hipLaunchKernelGGL(k_pair_w<true>, dim3(np), dim3(VT), 0, s, pairs, part, nchunk, W, flags,
                        cvg, strat0, s0steps, nswp, definite, nplus);
   else
-    hipLaunchKernelGGL(k_pair_w<false>, dim3(np), dim3(256), 0, s, pairs, part, nchunk, W, flags,
+    hipLaunchKernelGGL(k_pair_w<false>, dim3(np), dim3(VT), 0, s, pairs, part, nchunk, W, flags,
                        cvg, strat0, s0steps, nswp, definite, nplus);
 }
 
