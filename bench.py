@@ -14,9 +14,9 @@ oracle, is timed beside it on the same matrix (`plain`).  Accuracy is
 reported per singular value against the plain FP64 run, next to the spread
 between two FP64 runs that differ only in their strategy (cycwor against
 rowcyc), and the same for the hyperbolic SVD of the same matrix with
-NPLUS = N/2 (`hsvd`).  `wide32`: the plain FP64 method on 32-column blocks
-(64-column block pairs, step-by-step launches), SVD and HSVD, against the
-plain form at the reference's 16.  The CPU baseline is LAPACK dgesvj (J = I) on all
+NPLUS = N/2 (`hsvd`).  `wide32`: the plain FP64 method and the learned
+pair-skip on 32-column blocks (64-column block pairs, step-by-step
+launches), SVD and HSVD, against the plain form at the reference's 16.  The CPU baseline is LAPACK dgesvj (J = I) on all
 host cores, measured at two smaller orders of the same construction and
 extrapolated to N with the measured growth.
 
@@ -303,16 +303,25 @@ def main():
         # the plain FP64 method on 32-column blocks (64-column pairs), step by
         # step, against the plain form at the reference's 16 (cmp_run)
         wp = timed("plain", 1, 0, width=32)
+        wl = timed("learned", 1, 0, width=32)
         wide = {"block_width": 32, "form": "plain",
                 "s": wp["per"], "sweeps": wp["sweeps"],
                 "plain_width16": {"s": cmp_run["per"], "sweeps": cmp_run["sweeps"]},
                 "speedup_vs_plain_width16": cmp_run["per"] / wp["per"],
-                "per_sigma_vs_plain_width16": per_sigma(wp["D"], cmp_run["D"])}
+                "per_sigma_vs_plain_width16": per_sigma(wp["D"], cmp_run["D"]),
+                "learned": {"s": wl["per"], "sweeps": wl["sweeps"], "visits_skipped": wl["skipped"],
+                            "speedup_vs_plain_width16": cmp_run["per"] / wl["per"],
+                            "per_sigma_vs_plain_width16": per_sigma(wl["D"], cmp_run["D"])}}
         if hsvd is not None:
             wh = timed("plain", 1, 0, nplus=n // 2, width=32)
+            whl = timed("learned", 1, 0, nplus=n // 2, width=32)
             wide["hsvd"] = {"NPLUS": n // 2, "s": wh["per"], "sweeps": wh["sweeps"],
                             "speedup_vs_plain_width16": hsvd["plain"]["s"] / wh["per"],
-                            "per_sigma_vs_plain_width16": per_sigma(wh["D"], hp["D"])}
+                            "per_sigma_vs_plain_width16": per_sigma(wh["D"], hp["D"]),
+                            "learned": {"s": whl["per"], "sweeps": whl["sweeps"],
+                                        "visits_skipped": whl["skipped"],
+                                        "speedup_vs_plain_width16": hsvd["plain"]["s"] / whl["per"],
+                                        "per_sigma_vs_plain_width16": per_sigma(whl["D"], hp["D"])}}
 
     per = main_run["per"]
     pk = prof_run if prof_run is not None else main_run

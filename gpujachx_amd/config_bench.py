@@ -13,7 +13,8 @@ than the headline one, for the plain FP64 form and the learned+split form.
 
     python -m gpujachx_amd.config_bench c1 c2 [--forms plain learned+split plain32]
 
-(plain32: the plain FP64 method on 32-column blocks, 64-column pairs.)
+(plain32 / learned32: the plain FP64 method / the learned pair-skip on
+32-column blocks, 64-column pairs.)
 
 One JSON line per (configuration, form): s per call (device-resident
 input, after one warm-up call for the small configurations), sweeps,
@@ -139,12 +140,13 @@ def main():
         g.init_strats_builtin("cycwor", 32, "cycwor", cfg["n"] // 16)
         base = None
         for form in args.forms:
-            # plain32: the plain FP64 method on 32-column blocks
-            width = 32 if form == "plain32" else 16
+            # plain32 / learned32: the plain FP64 method / the learned pair-skip
+            # on 32-column blocks
+            width = 32 if form.endswith("32") else 16
             g.lib().hj_set_block_width(width)
             try:
-                t, W, V, D, sweeps, rb, sk = run(g, torch, cfg, Gt,
-                                                 "plain" if width == 32 else form, args.model)
+                t, W, V, D, sweeps, rb, sk = run(g, torch, cfg, Gt, form[:-2] if width == 32 else form,
+                                                 args.model)
             finally:
                 g.lib().hj_set_block_width(16)
             sig = torch.sqrt(D.abs())

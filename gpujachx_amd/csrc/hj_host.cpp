@@ -786,8 +786,11 @@ static void level2_core(unsigned routine, unsigned nrow, unsigned ncol, unsigned
 // families loaded by init_strats at the orders this width needs (64 for
 // level 0, ncol/32 for level 1, from the library's generators), then per
 // sweep every step's Gram partials, visits and block updates (G, and V with
-// ALG & 2), and the reference's stopping test (1/HypJacL2.cu:167).  Plain
-// FP64 only: the forms and the QR factorization act at width 16.
+// ALG & 2), and the reference's stopping test (1/HypJacL2.cu:167).  With
+// the learned pair-skip form the sweep starts with the plan of its visits
+// (the 16-column form's model and run-level policy, the visit records in
+// its units) and ends the run only when unskipped; the split Gram and the
+// QR factorization act at width 16 only.
 static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned nplus, double *dG,
                         unsigned lddG, double *dV, unsigned lddV, double *dD, unsigned *glbSwp,
                         unsigned long long *glb_s, unsigned long long *glb_b, hipStream_t caller)
@@ -818,14 +821,42 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   std::vector<uint8_t> s0(t0.tab.begin(), t0.tab.end());
   double *d_part = nullptr, *d_W = nullptr;
   unsigned *d_flags = nullptr;
-  unsigned long long *d_cvg = nullptr, *h_cvg = nullptr;
+  RunCtl *d_ctl = nullptr, *h_ctl = nullptr;
+  VisitRec *d_rec = nullptr;
+  uint8_t *d_plan = nullptr;
+  uint16_t *d_model = nullptr;
   uint8_t *d_s0 = nullptr;
   uint16_t *d_s1 = nullptr;
   HIP_CALL(hipMalloc(&d_part, sizeof(double) * (size_t)np * nchunk * wide_part_doubles()));
   HIP_CALL(hipMalloc(&d_W, sizeof(double) * (size_t)np * 4096u));
   HIP_CALL(hipMalloc(&d_flags, sizeof(unsigned) * np));
-  HIP_CALL(hipMalloc(&d_cvg, sizeof(unsigned long long)));
-  HIP_CALL(hipHostMalloc(&h_cvg, sizeof(unsigned long long)));
+  // the learned pair-skip at this width: the model and run-level policy of
+  // the 16-column form, the records in its units
+  const size_t nkeys = (size_t)np * steps;
+  const bool learned = (g_form & HJ_FORM_LEARNED_SKIP) && g_model_ready &&
+                       nkeys >= g_skip_min_visits;
+  HIP_CALL(hipMalloc(&d_ctl, sizeof(RunCtl)));
+  HIP_CALL(hipHostMalloc(&h_ctl, sizeof(RunCtl), hipHostMallocDefault));
+  memset(h_ctl, 0, sizeof(RunCtl));
+  const float max_rot = (float)nkeys * 2016.0f;
+  if (learned) {
+    HIP_CALL(hipMalloc(&d_rec, sizeof(VisitRec) * nkeys));
+    HIP_CALL(hipMemsetAsync(d_rec, 0, sizeof(VisitRec) * nkeys, s));
+    HIP_CALL(hipMalloc(&d_plan, nkeys));
+    HIP_CALL(hipMalloc(&d_model, sizeof(uint16_t) * MLP_NPARAM));
+    HIP_CALL(hipMemcpyAsync(d_model, g_model_bf16, sizeof(uint16_t) * MLP_NPARAM,
+                            hipMemcpyHostToDevice, s));
+    h_ctl->learned = 1u;
+    h_ctl->threshold = g_skip_threshold;
+    h_ctl->max_skips = g_skip_max;
+    h_ctl->tau = g_split_tau;
+    h_ctl->min_frac = g_skip_min_frac * ((nplus && nplus < ncol) ? g_skip_floor_mixed : 1.0f);
+    h_ctl->min_skip_frac = g_skip_min_propose;
+  }
+  h_ctl->max_rot = max_rot;
+  HIP_CALL(hipMemcpyAsync(d_ctl, h_ctl, sizeof(RunCtl), hipMemcpyHostToDevice, s));
+  HIP_CALL(hipStreamSynchronize(s));
+  const float log2n1 = (float)std::log2((double)(ncol / 32u));
   HIP_CALL(hipMalloc(&d_s0, s0.size()));
   HIP_CALL(hipMalloc(&d_s1, sizeof(uint16_t) * t1.tab.size()));
   HIP_CALL(hipMemcpyAsync(d_s0, s0.data(), s0.size(), hipMemcpyHostToDevice, s));
@@ -846,22 +877,27 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   long long swp_tim = 0;
   stopwatch_reset(swp_tim);
   while (blk_swp < g_max_sweeps) {
-    HIP_CALL(hipMemsetAsync(d_cvg, 0, sizeof(unsigned long long), s));
+    // counters, the sweep's switches (k_sweep_begin), the plan of its visits
+    launch_sweep_begin(s, d_ctl);
+    if (learned)
+      launch_plan(s, d_rec, (unsigned)nkeys, d_s1, nplus, log2n1, max_rot, d_ctl, d_model, true,
+                  d_plan, 32u);
     for (unsigned st = 0u; st < steps; ++st) {
       const uint16_t *const pairs = d_s1 + (size_t)st * np * 2u;
+      const uint8_t *const plan = learned ? d_plan + (size_t)st * np : nullptr;
       hipEvent_t *const ev = g_profile ? evs.data() + (size_t)4u * st : nullptr;
       if (ev) HIP_CALL(hipEventRecord(ev[0], s));
-      launch_gram_w(s, dG, lddG, pairs, np, nchunk, CH, d_part);
+      launch_gram_w(s, dG, lddG, pairs, np, nchunk, CH, d_part, plan);
       if (ev) HIP_CALL(hipEventRecord(ev[1], s));
-      launch_pair_w(s, cdsort, pairs, np, d_part, nchunk, d_W, d_flags, d_cvg, d_s0, t0.steps,
-                    nswp, definite, nplus);
+      launch_pair_w(s, cdsort, pairs, np, d_part, nchunk, d_W, d_flags, d_ctl, d_s0, t0.steps,
+                    nswp, definite, nplus, plan, learned ? d_rec + (size_t)st * np : nullptr);
       if (ev) HIP_CALL(hipEventRecord(ev[2], s));
       launch_update_w(s, dG, lddG, nrow, pairs, np, d_W, d_flags);
       if (V)
         launch_update_w(s, V, lddV, ncol, pairs, np, d_W, d_flags);
       if (ev) HIP_CALL(hipEventRecord(ev[3], s));
     }
-    HIP_CALL(hipMemcpyAsync(h_cvg, d_cvg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_CALL(hipMemcpyAsync(h_ctl, d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamSynchronize(s));
     if (g_profile)
       for (unsigned st = 0u; st < steps; ++st)
@@ -872,8 +908,9 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
           g_stats.kcnt[k] += 1ull;
         }
     ++blk_swp;
-    const unsigned cvg_s = static_cast<unsigned>(*h_cvg);
-    const unsigned cvg_b = static_cast<unsigned>(*h_cvg >> 32u);
+    const unsigned cvg_s = static_cast<unsigned>(h_ctl->cvg);
+    const unsigned cvg_b = static_cast<unsigned>(h_ctl->cvg >> 32u);
+    const unsigned skipped = h_ctl->skipped;
     *glb_s += cvg_s;
     *glb_b += cvg_b;
     const double tim_s = stopwatch_lap(swp_tim) * TS2S;
@@ -881,15 +918,19 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       g_stats.rot_s[blk_swp - 1u] = cvg_s;
       g_stats.rot_b[blk_swp - 1u] = cvg_b;
       g_stats.sec[blk_swp - 1u] = tim_s;
+      g_stats.skip[blk_swp - 1u] = skipped;
     }
     g_stats.nswp = blk_swp;
-    g_stats.visits += (unsigned long long)np * steps;
+    g_stats.skipped += skipped;
+    g_stats.visits += nkeys - skipped;
     if (!quiet()) {
       (void)fprintf(stdout, "BLK_SWP(%2u), ROT_S(%10u), ROT_B(%10u), TIME(%#12.6f s)\n", blk_swp,
                     cvg_s, cvg_b, tim_s);
       (void)fflush(stdout);
     }
-    if (!cvg_b)
+    // no big rotation ends the run only in an unskipped sweep; otherwise
+    // the next sweep certifies (k_sweep_begin turns skipping off for it)
+    if (!cvg_b && !skipped)
       break;
   }
   for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
@@ -900,8 +941,11 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   HIP_CALL(hipFree(d_part));
   HIP_CALL(hipFree(d_W));
   HIP_CALL(hipFree(d_flags));
-  HIP_CALL(hipFree(d_cvg));
-  HIP_CALL(hipHostFree(h_cvg));
+  HIP_CALL(hipFree(d_ctl));
+  HIP_CALL(hipHostFree(h_ctl));
+  if (d_rec) HIP_CALL(hipFree(d_rec));
+  if (d_plan) HIP_CALL(hipFree(d_plan));
+  if (d_model) HIP_CALL(hipFree(d_model));
   HIP_CALL(hipFree(d_s0));
   HIP_CALL(hipFree(d_s1));
 }

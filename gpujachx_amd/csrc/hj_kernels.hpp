@@ -171,12 +171,15 @@ void launch_initV(hipStream_t s, double *V, unsigned ifc, unsigned nRank, unsign
 // 64x64 per pair, the update in wide_update_chunks(nrow) workgroups a pair.
 unsigned wide_part_doubles();
 unsigned wide_update_chunks(unsigned nrow);
+// plan (null: every visit runs): the step's plans, skipped pairs get no Gram
+// and no visit; rec (null: none): the visits' records
 void launch_gram_w(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
-                   unsigned np, unsigned nchunk, unsigned CH, double *part);
+                   unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan);
 void launch_pair_w(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np,
                    const double *part, unsigned nchunk, double *W, unsigned *flags,
-                   unsigned long long *cvg, const uint8_t *strat0, unsigned s0steps,
-                   unsigned nswp, int definite, unsigned nplus);
+                   RunCtl *ctl, const uint8_t *strat0, unsigned s0steps,
+                   unsigned nswp, int definite, unsigned nplus, const uint8_t *plan,
+                   VisitRec *rec);
 void launch_update_w(hipStream_t s, double *M, unsigned ldM, unsigned nrow,
                      const uint16_t *pairs, unsigned np, const double *W,
                      const unsigned *flags);

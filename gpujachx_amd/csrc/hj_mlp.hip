@@ -63,16 +63,17 @@ __device__ void visit_features(const VisitRec &v, unsigned sweep_now, float prev
   f[11] = log2n1 / 10.0f;
 }
 
-// fraction of negative-J columns among the 32 columns of blocks (bi, bj)
-__device__ inline float pair_jmix(unsigned bi, unsigned bj, unsigned nplus)
+// fraction of negative-J columns among the 2 bw columns of blocks (bi, bj)
+// of bw columns each
+__device__ inline float pair_jmix(unsigned bi, unsigned bj, unsigned nplus, unsigned bw)
 {
   auto neg = [&](unsigned b) {
-    const unsigned lo = 16u * b, hi = lo + 16u;
-    if (nplus <= lo) return 16u;
+    const unsigned lo = bw * b, hi = lo + bw;
+    if (nplus <= lo) return bw;
     if (nplus >= hi) return 0u;
     return hi - nplus;
   };
-  return (float)(neg(bi) + neg(bj)) / 32.0f;
+  return (float)(neg(bi) + neg(bj)) / (float)(2u * bw);
 }
 
 // MLP forward on one feature vector with float weights in w (layout of
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, 
                                               float log2n1, float max_rot,
                                               RunCtl *__restrict__ ctl,
                                               const uint16_t *__restrict__ model, int learned,
-                                              uint8_t *__restrict__ plan)
+                                              uint8_t *__restrict__ plan, unsigned bw)
 {
   __shared__ float w[MLP_NPARAM];
   if (learned) {
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void k_plan(const VisitRec *__restrict__ rec, 
   if (learned && ctl->skip_on && v.sweep != 0u && v.skips < ctl->max_skips) {
     const float prev = (float)(unsigned)(ctl->prev_cvg >> 32) / max_rot;
     float f[MLP_F];
-    visit_features(v, ctl->sweep, prev, pair_jmix(strat1[2u * key], strat1[2u * key + 1u], nplus),
+    visit_features(v, ctl->sweep, prev, pair_jmix(strat1[2u * key], strat1[2u * key + 1u], nplus, bw),
                    log2n1, f);
     const float z = mlp_logit(w, f);
     const float p = 1.0f / (1.0f + expf(-z));
@@ -167,7 +168,7 @@ __global__ void k_features(const VisitRec *__restrict__ rec, unsigned nkeys,
   if (key >= nkeys)
     return;
   visit_features(rec[key], sweep_now, prev_frac,
-                 pair_jmix(strat1[2u * key], strat1[2u * key + 1u], nplus), log2n1,
+                 pair_jmix(strat1[2u * key], strat1[2u * key + 1u], nplus, 16u), log2n1,
                  X + (size_t)key * MLP_F);
 }
 
@@ -268,10 +269,10 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__re
 
 void launch_plan(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
                  unsigned nplus, float log2n1, float max_rot, RunCtl *ctl,
-                 const uint16_t *model, bool learned, uint8_t *plan)
+                 const uint16_t *model, bool learned, uint8_t *plan, unsigned bw)
 {
   hipLaunchKernelGGL(k_plan, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, rec, nkeys, strat1, nplus,
-                     log2n1, max_rot, ctl, model, learned ? 1 : 0, plan);
+                     log2n1, max_rot, ctl, model, learned ? 1 : 0, plan, bw);
   if (learned)
     hipLaunchKernelGGL(k_plan_gate, dim3((nkeys + 255u) / 256u), dim3(256), 0, s, nkeys,
                        (const RunCtl *)ctl, plan);

@@ -20,9 +20,10 @@ constexpr unsigned MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
 constexpr unsigned MLP_OFF_B3 = MLP_OFF_W3 + MLP_H;
 constexpr unsigned MLP_NPARAM = MLP_OFF_B3 + 1u;
 
+// bw: the block width of the pairs (16, or 32 for 64-column pairs)
 void launch_plan(hipStream_t s, const VisitRec *rec, unsigned nkeys, const uint16_t *strat1,
                  unsigned nplus, float log2n1, float max_rot, RunCtl *ctl,
-                 const uint16_t *model, bool learned, uint8_t *plan);
+                 const uint16_t *model, bool learned, uint8_t *plan, unsigned bw = 16u);
 
 }  // namespace hj
 

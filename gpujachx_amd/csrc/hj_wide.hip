@@ -59,11 +59,14 @@ struct GramLdsW {
 __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, unsigned ldG,
                                                 const uint16_t *__restrict__ pairs,
                                                 unsigned nchunk, unsigned CH,
-                                                double *__restrict__ part)
+                                                double *__restrict__ part,
+                                                const uint8_t *__restrict__ plan)
 {
   __shared__ GramLdsW S;
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const unsigned pi = blockIdx.x / nchunk, c = blockIdx.x - pi * nchunk;
+  if (plan && plan[pi] == PLAN_SKIP)
+    return;  // a skipped visit needs no Gram
   const unsigned a = pairs[2u * pi], b = pairs[2u * pi + 1u];
   const unsigned SUB = CH / nw;
   const size_t r0 = (size_t)c * CH + (size_t)w * SUB + (l & 15u);
@@ -227,6 +230,8 @@ struct PairLdsW {
   double W[64u * LDRW];
   double col[64];
   unsigned cnt_s[32], cnt_b[32];
+  float cmax[32];
+  float bnorm[2];
   int swp;
   uint8_t S0[S0W_BYTES];
 };
@@ -239,13 +244,24 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
                                                 const double *__restrict__ part, unsigned nchunk,
                                                 double *__restrict__ Wout,
                                                 unsigned *__restrict__ flags,
-                                                unsigned long long *__restrict__ cvg,
+                                                RunCtl *__restrict__ ctl,
                                                 const uint8_t *__restrict__ strat0,
                                                 unsigned s0steps, unsigned nswp, int definite,
-                                                unsigned nplus)
+                                                unsigned nplus, const uint8_t *__restrict__ plan,
+                                                VisitRec *__restrict__ rec)
 {
   __shared__ PairLdsW L;
   const unsigned tid = threadIdx.x, pi = blockIdx.x;
+  unsigned long long *const cvg = &ctl->cvg;
+  if (plan && plan[pi] == PLAN_SKIP) {
+    // skipped visit: no update, the record keeps the last executed visit
+    if (tid == 0u) {
+      flags[pi] = 0u;
+      rec[pi].skips += 1u;
+      atomicAdd(&ctl->skipped, 1u);
+    }
+    return;
+  }
   double *const R = L.R;
   double *const W = L.W;
   const unsigned iBlk = pairs[2u * pi], jBlk = pairs[2u * pi + 1u];
@@ -294,12 +310,21 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
   for (unsigned e = tid; e < 4096u; e += VT)
     RW(W, e & 63u, e >> 6) = ((e & 63u) == (e >> 6)) ? 1.0 : 0.0;
   __syncthreads();
+  if (rec && tid < 2u) {
+    // squared Frobenius norms of the two blocks, for the visit record
+    double sn = 0.0;
+    for (unsigned k = 0u; k < 32u; ++k)
+      sn = sn + RW(R, 32u * tid + k, 32u * tid + k);
+    L.bnorm[tid] = (float)log2(sn);
+  }
+  __syncthreads();
   chol64_block(R, L.col, tid);
   __syncthreads();
 
   const unsigned y = tid >> 4, r = tid & 15u;
   unsigned blk_s = 0u, blk_b = 0u;
   bool swapped = false;
+  double my_cos = 0.0;  // max |Apq| / sqrt(Dp Dq) of the slot's pivots, first inner sweep
   for (unsigned swp = 0u; swp < nswp; ++swp) {
     unsigned my_s = 0u, my_b = 0u;
     for (unsigned step = 0u; step < s0steps; ++step) {
@@ -322,6 +347,8 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
       const double Apq = sum16x4(t2);
       const double Dp_ = sqrt(Dp), Dq_ = sqrt(Dq), Apq_ = fabs(Apq), Dpq_ = Dp_ * Dq_;
       const int transf_s = !(Apq_ < (Dpq_ * WTOL));
+      if (rec && swp == 0u && Dpq_ > 0.0)
+        my_cos = fmax(my_cos, Apq_ / Dpq_);
       int pdef = def;
       if (hyp) {
         if ((p < npos) && (q < npos)) pdef = +1;
@@ -387,6 +414,8 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
   }
   if (swapped)
     L.swp = 1;
+  if (r == 0u)
+    L.cmax[y] = (float)my_cos;
   __syncthreads();
   if (tid == 0u) {
     // convergence counter, share/device_code_nosort.hpp:94-103
@@ -397,6 +426,23 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
         atomicAdd(reinterpret_cast<unsigned *>(cvg), blk_s);
     }
     flags[pi] = (blk_s || L.swp) ? 1u : 0u;
+    if (rec) {
+      // the record in the units of a 32-column pair's inner sweep (496 pivot
+      // pairs): this pair's counts scaled from its 2016
+      float cm = 0.0f;
+      for (unsigned i = 0u; i < 32u; ++i)
+        cm = fmaxf(cm, L.cmax[i]);
+      VisitRec v;
+      v.rot_s = (min(blk_s, 2016u) * 496u + 1008u) / 2016u;
+      v.rot_b = (min(blk_b, 2016u) * 496u + 1008u) / 2016u;
+      v.maxcos = cm;
+      v.l2ni = L.bnorm[0];
+      v.l2nj = L.bnorm[1];
+      v.sweep = ctl->sweep;
+      v.skips = 0u;
+      v.flags = 0u;
+      rec[pi] = v;
+    }
   }
   double *const wo = Wout + (size_t)pi * 4096u;
   for (unsigned e = tid; e < 4096u; e += VT)
@@ -479,24 +525,25 @@ unsigned wide_part_doubles() { return WPART; }
 unsigned wide_update_chunks(unsigned nrow) { return (nrow + UPW_ROWS - 1u) / UPW_ROWS; }
 
 void launch_gram_w(hipStream_t s, const double *G, unsigned ldG, const uint16_t *pairs,
-                   unsigned np, unsigned nchunk, unsigned CH, double *part)
+                   unsigned np, unsigned nchunk, unsigned CH, double *part, const uint8_t *plan)
 {
   const unsigned nw = gram_waves(CH);
   hipLaunchKernelGGL(k_gram_w, dim3(np * nchunk), dim3(64u * nw), 0, s, G, ldG, pairs, nchunk, CH,
-                     part);
+                     part, plan);
 }
 
 void launch_pair_w(hipStream_t s, bool sort, const uint16_t *pairs, unsigned np,
                    const double *part, unsigned nchunk, double *W, unsigned *flags,
-                   unsigned long long *cvg, const uint8_t *strat0, unsigned s0steps,
-                   unsigned nswp, int definite, unsigned nplus)
+                   RunCtl *ctl, const uint8_t *strat0, unsigned s0steps,
+                   unsigned nswp, int definite, unsigned nplus, const uint8_t *plan,
+                   VisitRec *rec)
 {
   if (sort)
     hipLaunchKernelGGL(k_pair_w<true>, dim3(np), dim3(VT), 0, s, pairs, part, nchunk, W, flags,
-                       cvg, strat0, s0steps, nswp, definite, nplus);
+                       ctl, strat0, s0steps, nswp, definite, nplus, plan, rec);
   else
     hipLaunchKernelGGL(k_pair_w<false>, dim3(np), dim3(VT), 0, s, pairs, part, nchunk, W, flags,
-                       cvg, strat0, s0steps, nswp, definite, nplus);
+                       ctl, strat0, s0steps, nswp, definite, nplus, plan, rec);
 }
 
 void launch_update_w(hipStream_t s, double *M, unsigned ldM, unsigned nrow,

@@ -66,6 +66,47 @@ def test_wide_larger(alg, nplus):
     assert_same(rg, ro, "1024x512 alg %d nplus %d" % (alg, nplus))
 
 
+@pytest.mark.parametrize("kind", ["svd", "hyp"])
+def test_wide_learned_skip(kind):
+    # the learned pair-skip on 32-column blocks (the packaged model, the
+    # run-level floors off): visits skipped, the run ends on an unskipped
+    # sweep without a big rotation, the spectrum per value as the plain run's
+    import os
+    import gpujachx_amd.learned as lr
+    from conftest import assert_per_value
+    if not os.path.exists(lr.DEFAULT_MODEL):
+        pytest.skip("no trained model file")
+    n = 1024
+    s = 10.0 ** (-10.0 * np.random.default_rng(3).uniform(size=n))
+    G = svd_input(n, n, s, seed=31)
+    nplus = n if kind == "svd" else n // 2
+    old = g.set_max_sweeps(80)
+    try:
+        g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+        rp = g.hypJacL2(9, G, nplus)
+        g.init_strats_builtin("cycwor", 32, "rowcyc", n // 16)
+        rr = g.hypJacL2(9, G, nplus)
+        g.init_strats_builtin("cycwor", 32, "cycwor", n // 16)
+        lr.enable(form=g.FORM_LEARNED_SKIP)
+        g.lib().hj_set_skip_min_visits(0)
+        g.lib().hj_set_skip_min_propose(0.0)
+        print("case: wide learned %s" % kind, flush=True)
+        rl = g.hypJacL2(9, G, nplus)
+        sk, skipped, visits = g.last_skips()
+        _, rb, _ = g.last_sweeps()
+    finally:
+        g.set_form(g.FORM_PLAIN)
+        g.set_max_sweeps(old)
+        g.lib().hj_set_skip_min_visits(32000)
+        g.lib().hj_set_skip_min_propose(0.1)
+    assert rp.ret == 0 and rl.ret == 0
+    print("plain %d sweeps, learned %d sweeps, %d of %d visits skipped"
+          % (rp.sweeps, rl.sweeps, skipped, skipped + visits))
+    assert skipped > 0
+    assert rb[-1] == 0 and sk[-1] == 0
+    assert_per_value(rl.D, rp.D, rr.D, "wide learned %s" % kind)
+
+
 def test_wide_padded_leading_dimension():
     G = graded_input(320, 128, seed=9)
     print("case: wide ldhG 400", flush=True)
