@@ -876,7 +876,11 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
   unsigned blk_swp = 0u;
   long long swp_tim = 0;
   stopwatch_reset(swp_tim);
-  while (blk_swp < g_max_sweeps) {
+  // one sweep as a graph (start-of-sweep kernel, plan, every step, the
+  // control block to pinned memory), launched once a sweep; with profiling
+  // the launches go one by one with events around them
+  hipGraphExec_t gexec = nullptr;
+  auto sweep = [&]() {
     // counters, the sweep's switches (k_sweep_begin), the plan of its visits
     launch_sweep_begin(s, d_ctl);
     if (learned)
@@ -898,6 +902,20 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
       if (ev) HIP_CALL(hipEventRecord(ev[3], s));
     }
     HIP_CALL(hipMemcpyAsync(h_ctl, d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, s));
+  };
+  if (!g_profile && !getenv("HJ_NOGRAPH")) {
+    hipGraph_t graph;
+    HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    sweep();
+    HIP_CALL(hipStreamEndCapture(s, &graph));
+    HIP_CALL(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    HIP_CALL(hipGraphDestroy(graph));
+  }
+  while (blk_swp < g_max_sweeps) {
+    if (gexec)
+      HIP_CALL(hipGraphLaunch(gexec, s));
+    else
+      sweep();
     HIP_CALL(hipStreamSynchronize(s));
     if (g_profile)
       for (unsigned st = 0u; st < steps; ++st)
@@ -933,6 +951,7 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     if (!cvg_b && !skipped)
       break;
   }
+  if (gexec) HIP_CALL(hipGraphExecDestroy(gexec));
   for (auto &e : evs) HIP_CALL(hipEventDestroy(e));
   *glbSwp = blk_swp;
   launch_initD(s, dG, dD, 0u, nrow, ncol, nplus, lddG);
