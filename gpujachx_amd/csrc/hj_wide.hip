@@ -450,37 +450,35 @@ __global__ __launch_bounds__(512) void k_pair_w(const uint16_t *__restrict__ pai
 }
 
 // Block update M[:, pair] <- M[:, pair] W of the pairs of one step: a
-// workgroup per (pair, 512-row chunk), each wave 16-row slabs of the chunk
-// with the next slab's loads in flight.  Lane l keeps its W operands in
-// registers, wv[4u + h] = W(4u + l/16, 16h + l%16).  Per slab the MFMA
-// chains run over k = 0..63 in order: A = W^T, B = the slab's transpose
-// (lane l: M(row + l%16, column 4u + l/16)); lane l, register q of tile h
-// holds output column 16h + 4q + l/16 of row row + l%16.  Pairs whose W is
-// the identity (flag 0) are left alone.
+// workgroup per (pair, 512-row chunk), W in LDS (row-major, padded), each
+// wave 16-row slabs of the chunk with the next slab's loads in flight.  Per
+// slab the MFMA chains run over k = 0..63 in order: A = W^T (lane l:
+// W(4u + l/16, 16h + l%16)), B = the slab's transpose (lane l:
+// M(row + l%16, column 4u + l/16)); lane l, register q of tile h holds
+// output column 16h + 4q + l/16 of row row + l%16.  Pairs whose W is the
+// identity (flag 0) are left alone.
 constexpr unsigned UPW_ROWS = 512u;
+constexpr unsigned LDWW = 65u;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_update_w(double *__restrict__ M, unsigned ldM,
-                                                  unsigned nrow, const uint16_t *__restrict__ pairs,
-                                                  const double *__restrict__ Wbuf,
-                                                  const unsigned *__restrict__ flags,
-                                                  unsigned nchunk)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_update_w(
+    double *__restrict__ M, unsigned ldM, unsigned nrow, const uint16_t *__restrict__ pairs,
+    const double *__restrict__ Wbuf, const unsigned *__restrict__ flags, unsigned nchunk)
 {
+  __shared__ double Wl[64u * LDWW];
   const unsigned pi = blockIdx.x / nchunk, c = blockIdx.x - pi * nchunk;
   if (!flags[pi])
     return;
   const unsigned l = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const unsigned a = pairs[2u * pi], b = pairs[2u * pi + 1u];
   const double *const Wp = Wbuf + (size_t)pi * 4096u;  // W(k, j) at 64 j + k
-  double wv[64];
-#pragma unroll
-  for (unsigned u = 0u; u < 16u; ++u)
-#pragma unroll
-    for (unsigned h = 0u; h < 4u; ++h)
-      wv[4u * u + h] = Wp[(16u * h + (l & 15u)) * 64u + 4u * u + (l >> 4)];
+  for (unsigned e = threadIdx.x; e < 4096u; e += blockDim.x)
+    Wl[(e & 63u) * LDWW + (e >> 6)] = Wp[e];  // Wl[k * LDWW + j] = W(k, j)
+  __syncthreads();
   const unsigned rbeg = c * UPW_ROWS, rend = min(rbeg + UPW_ROWS, nrow);
   // column 4u + l/16 of the pair: block a for u < 8, block b after
   double *const ca = M + (size_t)(32u * a + (l >> 4)) * ldM + (l & 15u);
   double *const cb = M + (size_t)(32u * b + (l >> 4)) * ldM + (l & 15u);
+  const double *const wl = Wl + (l >> 4) * LDWW + (l & 15u);
   unsigned row = rbeg + 16u * w;
   double xn[16];
   if (row < rend) {
@@ -506,11 +504,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (unsigned h = 0u; h < 4u; ++h)
       acc[h] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (unsigned u = 0u; u < 16u; ++u)
+#pragma unroll 4
+    for (unsigned u = 0u; u < 16u; ++u) {
+      const double *const wr = wl + 4u * u * LDWW;
 #pragma unroll
       for (unsigned h = 0u; h < 4u; ++h)
-        acc[h] = mfma(wv[4u * u + h], x[u], acc[h]);
+        acc[h] = mfma(wr[16u * h], x[u], acc[h]);
+    }
 #pragma unroll
     for (unsigned h = 0u; h < 4u; ++h)
 #pragma unroll
