@@ -51,12 +51,12 @@ constexpr unsigned LDTW = 17u;  // padded column of a wave's slab buffer
 
 struct GramLdsW {
   union {
-    d4 red[GRAM_WAVES - 1][WTILES][64];
+    d4 red[WTILES][64];  // one wave's tiles at a time
     double T[GRAM_WAVES][64u * LDTW];
   };
 };
 
-__global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, unsigned ldG,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gram_w(const double *__restrict__ G, unsigned ldG,
                                                 const uint16_t *__restrict__ pairs,
                                                 unsigned nchunk, unsigned CH,
                                                 double *__restrict__ part,
@@ -113,22 +113,27 @@ __global__ __launch_bounds__(256) void k_gram_w(const double *__restrict__ G, un
           acc[tt] = mfma(X[ti][r], X[tj][r], acc[tt]);
     }
   }
-  __syncthreads();  // the slab buffers become the reduction buffer
-  if (w) {
+  // wave 0 adds the other waves' tiles in wave order, one wave's at a time
+  // through the (former slab) buffer
+  for (unsigned u = 1u; u < nw; ++u) {
+    __syncthreads();
+    if (w == u) {
 #pragma unroll
-    for (unsigned tt = 0u; tt < WTILES; ++tt)
-      S.red[w - 1u][tt][l] = acc[tt];
+      for (unsigned tt = 0u; tt < WTILES; ++tt)
+        S.red[tt][l] = acc[tt];
+    }
+    __syncthreads();
+    if (w == 0u) {
+#pragma unroll
+      for (unsigned tt = 0u; tt < WTILES; ++tt)
+        acc[tt] = acc[tt] + S.red[tt][l];
+    }
   }
-  __syncthreads();
   if (w == 0u) {
     double *const o = part + (size_t)blockIdx.x * WPART;
 #pragma unroll
-    for (unsigned tt = 0u; tt < WTILES; ++tt) {
-      d4 v = acc[tt];
-      for (unsigned u = 1u; u < nw; ++u)
-        v = v + S.red[u - 1u][tt][l];
-      *reinterpret_cast<d4 *>(o + tt * 256u + 4u * l) = v;
-    }
+    for (unsigned tt = 0u; tt < WTILES; ++tt)
+      *reinterpret_cast<d4 *>(o + tt * 256u + 4u * l) = acc[tt];
   }
 }
 
