@@ -281,13 +281,16 @@ unsigned hj_gram_nchunk(unsigned nrow, unsigned npairs);
 /* Level-2 sweep cap (the reference's compile-time HYPJAC_NSWEEP, 30 by
  * default), 1..256; returns the previous cap.  Inner sweeps (ALG < 8) keep
  * the reference's cap of 30. */
-/* Block width of the following calls: 16 (default, the reference's
- * HYPJACL1_NCOLB, share/HypJacL.hpp:112-116) or 32 (64-column block pairs:
- * a 64x64 Gram, Cholesky and level-0 per visit, half the HBM bytes per
- * sweep; plain FP64 and the Cholesky path only; the strategy families
- * loaded by init_strats are generated at orders 64 and ncol/32: -3 when ncol
- * is not a multiple of 64 or a family has no table of such an order).
- * Other values are ignored.  Returns the previous width. */
+/* Block width of the following hypJacL2 / hypJacL2_dev calls: 16 (default,
+ * the reference's HYPJACL1_NCOLB, share/HypJacL.hpp:112-116) or 32
+ * (64-column block pairs: a 64x64 Gram, Cholesky and level-0 per visit,
+ * half the HBM bytes per sweep).  At 32 the plain FP64 method and the
+ * learned pair-skip run; the split Gram bit of the form and the QR / Drmac
+ * factorization are ignored (Cholesky, FP64 Grams).  The strategy families
+ * loaded by init_strats are generated at orders 64 and ncol/32: -3 when
+ * ncol is not a multiple of 64 or a family has no table of such an order.
+ * hypJacL2_dev_noalloc and the level-1 layer keep width 16.  Other values
+ * are ignored.  Returns the previous width. */
 unsigned hj_set_block_width(unsigned width);
 unsigned hj_set_max_sweeps(unsigned n);
 
