@@ -107,6 +107,28 @@ def test_wide_learned_skip(kind):
     assert_per_value(rl.D, rp.D, rr.D, "wide learned %s" % kind)
 
 
+def test_wide_ignores_split_and_qr():
+    # at width 32 the split-Gram bit of the form and the QR factorization
+    # are ignored: the plain FP64 run's bits
+    G = graded_input(512, 256, seed=21)
+    g.init_strats_builtin("cycwor", 32, "cycwor", 256 // 16)
+    rp = g.hypJacL2(11, G, 100)
+    g.set_form(g.FORM_SPLIT_GRAM)
+    try:
+        rs = g.hypJacL2(11, G, 100)
+    finally:
+        g.set_form(g.FORM_PLAIN)
+    old = g.set_factorization(g.FACT_QR)
+    try:
+        rq = g.hypJacL2(11, G, 100)
+    finally:
+        g.set_factorization(old)
+    for r in (rs, rq):
+        assert r.ret == 0 and (r.sweeps, r.rot_s, r.rot_b) == (rp.sweeps, rp.rot_s, rp.rot_b)
+        for a, b in ((r.G, rp.G), (r.V, rp.V), (r.D, rp.D)):
+            np.testing.assert_array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
 def test_wide_padded_leading_dimension():
     G = graded_input(320, 128, seed=9)
     print("case: wide ldhG 400", flush=True)
