@@ -17,6 +17,8 @@
 //     butterfly 32,16 in the lane, 8,4,2,1 across lanes; the convergence
 //     test at tolerance sqrt(64) 2^-53;
 //   - update: out(r, j) = MFMA chain over k = 0..63.
+// With the learned pair-skip a step's plan marks skipped pairs: no Gram, no
+// visit (its record keeps the last executed visit), no update.
 #include "../../include/hypjac.h"
 #include "hj_kernels.hpp"
 #include "hj_device.hpp"
