@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--model", default=None)
     ap.add_argument("--skip-min-frac", type=float, default=None)
     ap.add_argument("--skip-floor-mixed", type=float, default=None,
-                    help="floor multiple for runs that mix J signs (default 3)")
+                    help="floor multiple for runs that mix J signs (default 3, 6 at width 32)")
     args = ap.parse_args()
     os.environ.setdefault("HJ_QUIET", "1")
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -144,6 +144,8 @@ def main():
             # on 32-column blocks
             width = 32 if form.endswith("32") else 16
             g.lib().hj_set_block_width(width)
+            if args.skip_floor_mixed is not None:
+                g.lib().hj_set_skip_floor_mixed(args.skip_floor_mixed)
             try:
                 t, W, V, D, sweeps, rb, sk = run(g, torch, cfg, Gt, form[:-2] if width == 32 else form,
                                                  args.model)

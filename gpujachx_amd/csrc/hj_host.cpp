@@ -56,8 +56,11 @@ static int g_fused_split = 0;
 // fraction of the most a sweep can have (the last sweeps run unskipped)
 static float g_skip_min_frac = 0.01f;
 // runs that mix J signs skip only above this multiple of the floor (their
-// skips cost more extra sweeps: N = 8192 and 16384, measured)
+// skips cost more extra sweeps: N = 8192 and 16384, measured), per block
+// width (the big-rotation fraction of a 64-column pair's sweep is counted
+// against 2016 pivot pairs)
 static float g_skip_floor_mixed = 3.0f;
+static float g_skip_floor_mixed_w32 = 6.0f;
 // ... and only in sweeps where the model proposes at least this fraction of
 // all visits
 static float g_skip_min_propose = 0.1f;
@@ -850,7 +853,7 @@ static void level2_wide(unsigned routine, unsigned nrow, unsigned ncol, unsigned
     h_ctl->threshold = g_skip_threshold;
     h_ctl->max_skips = g_skip_max;
     h_ctl->tau = g_split_tau;
-    h_ctl->min_frac = g_skip_min_frac * ((nplus && nplus < ncol) ? g_skip_floor_mixed : 1.0f);
+    h_ctl->min_frac = g_skip_min_frac * ((nplus && nplus < ncol) ? g_skip_floor_mixed_w32 : 1.0f);
     h_ctl->min_skip_frac = g_skip_min_propose;
   }
   h_ctl->max_rot = max_rot;
@@ -1370,7 +1373,10 @@ extern "C" int hj_set_fused_split(int on)
 extern "C" void hj_set_skip_min_frac(float frac) { g_skip_min_frac = frac; }
 
 // The floor of runs that mix J signs, as a multiple of hj_set_skip_min_frac's
-extern "C" void hj_set_skip_floor_mixed(float factor) { g_skip_floor_mixed = factor; }
+extern "C" void hj_set_skip_floor_mixed(float factor)
+{
+  (g_width == 32u ? g_skip_floor_mixed_w32 : g_skip_floor_mixed) = factor;
+}
 
 extern "C" void hj_set_skip_min_propose(float frac) { g_skip_min_propose = frac; }
 

@@ -159,8 +159,9 @@ int hj_set_skip_model(const float *params, unsigned nparam, float threshold,
  * min_propose of all visits (default 0.1): a few skips cannot pay for the
  * sweeps they may add. */
 void hj_set_skip_min_frac(float frac);
-/* runs whose J mixes signs (0 < NPLUS < N) use factor x that floor
- * (default 3) */
+/* runs whose J mixes signs (0 < NPLUS < N) use factor x that floor; the
+ * factor of the block width in force (hj_set_block_width): default 3 at
+ * width 16, 6 at width 32 */
 void hj_set_skip_floor_mixed(float factor);
 void hj_set_skip_min_propose(float min_propose);
 /* Skipping only in runs with at least this many block-pair visits per sweep
